@@ -1,0 +1,156 @@
+"""Seeded synthetic workloads (molecules, basis sets, densities).
+
+Five workloads are defined, each named by what it is:
+
+* ``small8``   - 8 atoms uniform in [-3,3]^3 bohr (min separation 1.5 bohr);
+  per atom two s shells (3 primitives, exponents log-uniform [0.1, 20]) and one
+  p shell (2 primitives, log-uniform [0.1, 5]); contraction coefficients
+  U[0.1, 1]; nbf = 40; D = C C^T with C (40 x 10) i.i.d. N(0, 0.3^2).
+* ``lattice64`` - 64 atoms on a 4x4x4 cubic lattice (spacing 2.5 bohr, jitter
+  U[-0.3, 0.3] per coordinate); per atom 3 s (3 prims), 2 p (2 prims), 1
+  spherical d (1 prim, exponent log-uniform [0.3, 2]); nbf = 896 (960
+  Cartesian); C (896 x 224) N(0, 0.1^2).
+* ``lattice256`` - the same generator and basis on an 8x8x4 lattice: 256 atoms,
+  nbf = 3584, C (3584 x 896) N(0, 0.1^2).  (The task text says "basis as the
+  8-atom case" but quotes nbf = 3584 = 256 x 14, which only the lattice basis
+  gives; the lattice basis is used.)
+* ``batch`` - molecules with floor(log-uniform[4, 48]) atoms placed and
+  described like ``small8``; nocc = ceil(nbf / 4), sigma = 0.3.
+* ``serve`` - molecules of floor(log-uniform[8, 200]) atoms with the ``small8``
+  basis in a cube of edge 6 (n/8)^(1/3) bohr (keeps the 8-atom number
+  density: 200 atoms do not fit in [-3,3]^3 at 1.5 bohr spacing).
+
+Seeds: workload k uses ``numpy.random.default_rng(1000 + k)``; molecule m of the
+batch/serve workloads uses ``default_rng(1000 * k + m)``.  The workload ids are
+1..5 in the order above.
+"""
+
+from __future__ import annotations
+
+import math
+
+import numpy as np
+
+from .basis import Molecule, Shell
+
+WORKLOAD_IDS = {"small8": 1, "lattice64": 2, "lattice256": 3, "batch": 4, "serve": 5}
+
+
+def _log_uniform(rng, lo, hi, size=None):
+    return np.exp(rng.uniform(math.log(lo), math.log(hi), size=size))
+
+
+def random_positions(rng, natoms, edge=6.0, min_sep=1.5, max_tries=200000):
+    """Rejection sampling of ``natoms`` points uniform in a cube of edge ``edge``
+    centred at the origin with pairwise distance >= ``min_sep``."""
+    pts = []
+    tries = 0
+    half = 0.5 * edge
+    while len(pts) < natoms:
+        tries += 1
+        if tries > max_tries:
+            raise RuntimeError(f"could not place {natoms} atoms in edge {edge}")
+        p = rng.uniform(-half, half, size=3)
+        if all(np.linalg.norm(p - q) >= min_sep for q in pts):
+            pts.append(p)
+    return np.array(pts)
+
+
+def small_basis_shells(rng, atom, center):
+    """Two s shells (3 prims) and one p shell (2 prims) per atom."""
+    shells = []
+    for _ in range(2):
+        shells.append(Shell(atom, 0, center, _log_uniform(rng, 0.1, 20.0, 3), rng.uniform(0.1, 1.0, 3)))
+    shells.append(Shell(atom, 1, center, _log_uniform(rng, 0.1, 5.0, 2), rng.uniform(0.1, 1.0, 2)))
+    return shells
+
+
+def lattice_basis_shells(rng, atom, center):
+    """Three s (3 prims), two p (2 prims), one spherical d (1 prim) per atom."""
+    shells = []
+    for _ in range(3):
+        shells.append(Shell(atom, 0, center, _log_uniform(rng, 0.1, 20.0, 3), rng.uniform(0.1, 1.0, 3)))
+    for _ in range(2):
+        shells.append(Shell(atom, 1, center, _log_uniform(rng, 0.1, 5.0, 2), rng.uniform(0.1, 1.0, 2)))
+    shells.append(Shell(atom, 2, center, _log_uniform(rng, 0.3, 2.0, 1), rng.uniform(0.1, 1.0, 1), spherical=True))
+    return shells
+
+
+def density(rng, nbf, nocc, sigma):
+    C = rng.normal(0.0, sigma, size=(nbf, nocc))
+    return C @ C.T
+
+
+def small_molecule(rng, natoms, edge=6.0, name=""):
+    coords = random_positions(rng, natoms, edge=edge)
+    shells = []
+    for a in range(natoms):
+        shells += small_basis_shells(rng, a, coords[a])
+    return Molecule(coords, shells, name=name)
+
+
+def lattice_molecule(rng, dims, spacing=2.5, jitter=0.3, name=""):
+    nx, ny, nz = dims
+    coords = []
+    for ix in range(nx):
+        for iy in range(ny):
+            for iz in range(nz):
+                coords.append(np.array([ix, iy, iz], dtype=np.float64) * spacing)
+    coords = np.array(coords)
+    coords -= coords.mean(axis=0)
+    coords += rng.uniform(-jitter, jitter, size=coords.shape)
+    shells = []
+    for a in range(coords.shape[0]):
+        shells += lattice_basis_shells(rng, a, coords[a])
+    return Molecule(coords, shells, name=name)
+
+
+def make_small8():
+    rng = np.random.default_rng(1000 + WORKLOAD_IDS["small8"])
+    mol = small_molecule(rng, 8, name="small8")
+    D = density(rng, mol.nbf, 10, 0.3)
+    return mol, D
+
+
+def make_lattice64():
+    rng = np.random.default_rng(1000 + WORKLOAD_IDS["lattice64"])
+    mol = lattice_molecule(rng, (4, 4, 4), name="lattice64")
+    D = density(rng, mol.nbf, mol.nbf // 4, 0.1)
+    return mol, D
+
+
+def make_lattice256():
+    rng = np.random.default_rng(1000 + WORKLOAD_IDS["lattice256"])
+    mol = lattice_molecule(rng, (8, 8, 4), name="lattice256")
+    D = density(rng, mol.nbf, mol.nbf // 4, 0.1)
+    return mol, D
+
+
+def make_batch_molecule(m, wid=WORKLOAD_IDS["batch"], lo=4, hi=48, density_sigma=0.3):
+    rng = np.random.default_rng(1000 * wid + m)
+    n = int(math.floor(_log_uniform(rng, lo, hi)))
+    mol = small_molecule(rng, n, name=f"batch{m}")
+    D = density(rng, mol.nbf, int(math.ceil(mol.nbf / 4)), density_sigma)
+    return mol, D
+
+
+def make_serve_molecule(m):
+    wid = WORKLOAD_IDS["serve"]
+    rng = np.random.default_rng(1000 * wid + m)
+    n = int(math.floor(_log_uniform(rng, 8, 200)))
+    edge = 6.0 * (n / 8.0) ** (1.0 / 3.0)
+    mol = small_molecule(rng, n, edge=edge, name=f"serve{m}")
+    D = density(rng, mol.nbf, int(math.ceil(mol.nbf / 4)), 0.3)
+    return mol, D
+
+
+def serve_iteration_density(D0, m, it):
+    """Density of SCF iteration ``it`` of serve molecule ``m``: D0 + N(0, 1e-3^2),
+    symmetrised."""
+    rng = np.random.default_rng(10_000_000 + 1000 * m + it)
+    E = rng.normal(0.0, 1e-3, size=D0.shape)
+    return D0 + 0.5 * (E + E.T)
+
+
+def make_workload(name):
+    return {"small8": make_small8, "lattice64": make_lattice64, "lattice256": make_lattice256}[name]()

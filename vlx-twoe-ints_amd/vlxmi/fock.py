@@ -1,0 +1,234 @@
+"""J/K Fock build driver on one GPU.
+
+``FockEngine(mol)`` keeps everything that does not depend on the density
+resident on the device (primitive tables, screened pair lists with Schwarz
+factors, K CSR lists, (i,k) lists, Boys tables, assembly indices).  Each
+``build(D)`` call:
+
+1. transforms D to the Cartesian AO basis (T^T D T, on the device);
+2. gathers ``mat_D`` for every J pair list, computes the block bounds
+   ``beta = max|Q D|`` and orders the ket copy of each list by non-increasing
+   bound (device sort);
+3. launches the 36 ``computeCoulombFock{AB}{CD}`` and 54
+   ``computeExchangeFock{A}{B}{C}{D}`` kernels through the C ABI;
+4. assembles AO J and K from the per-pair outputs and transforms them back
+   (T J T^T), and returns J, K (and E_J = 1/2 tr(D J), E_K = 1/2 tr(D K)).
+
+VeloxChem's own host driver (pair building, sorting, AO assembly) is not in
+the reference tree; this is the minimal host the kernels need.
+"""
+
+from __future__ import annotations
+
+import ctypes as C
+import math
+import time
+
+import numpy as np
+
+from . import pairs as P
+from .basis import NCART
+from .runtime import Device, check, lib
+
+
+class FockEngine:
+    def __init__(self, mol, thr=1e-12, dev=None, d_bound=100.0, omega=0.0, target_waves=2048):
+        self.dev = dev or Device()
+        t = self.dev.torch
+        self.torch = t
+        self.mol = mol
+        self.thr = float(thr)
+        self.omega = float(omega)
+        self.target_waves = target_waves
+        t0 = time.time()
+        self.fl = P.build_lists(mol, self.dev.schwarz, thr=thr, d_bound=d_bound)
+        self.setup_lists_s = time.time() - t0
+        fl = self.fl
+        d = self.dev
+        self.naos = fl.naos
+        self.T = d.tensor(mol.cart2sph())
+        self.prim_info = [d.tensor(p.info) if p is not None else None for p in fl.prims]
+        self.prim_aoinds = [d.tensor(p.aoinds) if p is not None else None for p in fl.prims]
+        self.prim_count = [p.count if p is not None else 0 for p in fl.prims]
+        self.jl = {}
+        for pc, jl in fl.jlists.items():
+            self.jl[pc] = dict(
+                first=d.tensor(jl.first), second=d.tensor(jl.second),
+                pair_data=d.tensor(jl.pair_data), mat_Q=d.tensor(jl.mat_Q),
+                ao_first=d.tensor(jl.ao_first), ao_second=d.tensor(jl.ao_second),
+                keep=d.tensor(~jl.lower_diag), count=jl.count, nblk=jl.nblk,
+                nab=NCART[jl.la] * NCART[jl.lb])
+        self.kc = {}
+        for pc, cs in fl.kcsr.items():
+            self.kc[pc] = dict(displs=d.tensor(cs.displs), counts=d.tensor(cs.counts), Q=d.tensor(cs.Q),
+                               D_inds=d.tensor(cs.D_inds), pair_data=d.tensor(cs.pair_data))
+        self.ik = {}
+        for pc, ik in fl.iklists.items():
+            self.ik[pc] = dict(i=d.tensor(ik.i), k=d.tensor(ik.k), count=int(ik.i.size))
+        self._build_assembly()
+        # AO index sets per angular type (for maxD_bd)
+        self.ao_of_type = []
+        for p in fl.prims:
+            self.ao_of_type.append(d.tensor(np.unique(p.aoinds.ravel()).astype(np.int64)) if p is not None else None)
+        self.setup_s = time.time() - t0
+
+    # ------------------------------------------------------------------ assembly
+    def _build_assembly(self):
+        fl, d, n = self.fl, self.dev, self.fl.naos
+        self.jasm = {}
+        for pc, jl in fl.jlists.items():
+            use = ~jl.lower_diag
+            f, s = jl.ao_first, jl.ao_second
+            off = (jl.first != jl.second) if jl.la == jl.lb else np.ones(jl.count, dtype=bool)
+            sel1 = np.nonzero(use)[0]
+            sel2 = np.nonzero(use & off)[0]
+            self.jasm[pc] = (d.tensor(sel1.astype(np.int64)), d.tensor((f[sel1] * n + s[sel1]).astype(np.int64)),
+                             d.tensor(sel2.astype(np.int64)), d.tensor((s[sel2] * n + f[sel2]).astype(np.int64)))
+        self.kasm = {}
+        for pc, ik in fl.iklists.items():
+            la, lc = ik.la, ik.lc
+            na, nc = NCART[la], NCART[lc]
+            ai = fl.prims[la].aoinds[ik.i % na, ik.i // na].astype(np.int64)
+            ak = fl.prims[lc].aoinds[ik.k % nc, ik.k // nc].astype(np.int64)
+            if la != lc:
+                sel2 = np.arange(ik.i.size)
+            else:
+                Iblk = np.repeat(ik.I, na * nc)
+                Kblk = np.repeat(ik.K, na * nc)
+                sel2 = np.nonzero(Iblk != Kblk)[0]
+            self.kasm[pc] = (d.tensor(ai * n + ak), d.tensor(sel2.astype(np.int64)),
+                             d.tensor((ak[sel2] * n + ai[sel2]).astype(np.int64)))
+
+    def assemble_J(self, matJ):
+        t = self.torch
+        n = self.naos
+        J = t.zeros(n * n, dtype=t.float64, device=self.dev.device)
+        for pc, v in matJ.items():
+            s1, i1, s2, i2 = self.jasm[pc]
+            J.index_add_(0, i1, v.index_select(0, s1))
+            J.index_add_(0, i2, v.index_select(0, s2))
+        return J.view(n, n)
+
+    def assemble_K(self, matK):
+        t = self.torch
+        n = self.naos
+        K = t.zeros(n * n, dtype=t.float64, device=self.dev.device)
+        for pc, v in matK.items():
+            i1, s2, i2 = self.kasm[pc]
+            K.index_add_(0, i1, v)
+            K.index_add_(0, i2, v.index_select(0, s2))
+        return K.view(n, n)
+
+    # ------------------------------------------------------------------ per build
+    def cart_density(self, D_sph):
+        t = self.torch
+        if not t.is_tensor(D_sph):
+            D_sph = self.dev.tensor(np.asarray(D_sph, dtype=np.float64))
+        return (self.T.t() @ D_sph @ self.T).contiguous()
+
+    def prepare_kets(self, Dc):
+        """mat_D per J list and the ket-ordered copies (device)."""
+        t = self.torch
+        kets = {}
+        for pc, j in self.jl.items():
+            mD = Dc[j["ao_first"], j["ao_second"]] * j["keep"].to(t.float64)
+            nab = j["nab"]
+            beta = (j["mat_Q"] * mD).abs().view(-1, nab).amax(dim=1)
+            order = t.sort(beta, descending=True, stable=True).indices
+            ent = (order[:, None] * nab + t.arange(nab, device=Dc.device)[None, :]).reshape(-1)
+            kets[pc] = dict(first=j["first"][ent].contiguous(), second=j["second"][ent].contiguous(),
+                            pair_data=j["pair_data"][ent].contiguous(), mat_Q=j["mat_Q"][ent].contiguous(),
+                            mat_D=mD[ent].contiguous(), count=j["count"])
+        return kets
+
+    def _ket_split(self, nbra_blocks, nket_blocks):
+        tiles = max(1, (nbra_blocks + 63) // 64)
+        s = max(1, -(-self.target_waves // tiles))
+        return int(min(s, max(1, nket_blocks // 64)))
+
+    def run_J(self, Dc, kets=None, classes=None):
+        """Per bra class mat_J (bra list order), summed over ket classes."""
+        t = self.torch
+        L = lib()
+        kets = kets or self.prepare_kets(Dc)
+        out = {}
+        for ab, cd in (classes or P.j_classes()):
+            if ab not in self.jl or cd not in self.jl:
+                continue
+            bra, ket = self.jl[ab], kets[cd]
+            if ab not in out:
+                out[ab] = t.zeros(bra["count"], dtype=t.float64, device=Dc.device)
+            cls = ab + cd
+            args = [out[ab].data_ptr()]
+            for x in "spd":
+                if x in cls:
+                    l = P.LVAL[x]
+                    args += [self.prim_info[l].data_ptr(), self.prim_count[l]]
+            args += [ket["mat_D"].data_ptr(), bra["mat_Q"].data_ptr(), ket["mat_Q"].data_ptr(),
+                     bra["first"].data_ptr(), bra["second"].data_ptr(), bra["pair_data"].data_ptr(), bra["count"],
+                     ket["first"].data_ptr(), ket["second"].data_ptr(), ket["pair_data"].data_ptr(), ket["count"],
+                     self.dev.boys_table.data_ptr(), self.dev.boys_ft.data_ptr(), self.thr]
+            split = self._ket_split(bra["nblk"], self.jl[cd]["nblk"])
+            fn = getattr(L, "vlxmi_launch_computeCoulombFock" + cls.upper())
+            check(fn(*args, split, self.dev.stream_ptr()), "computeCoulombFock" + cls.upper())
+        return out
+
+    def max_D(self, Dc):
+        out = {}
+        for lb in range(3):
+            for ld in range(3):
+                ab, ad = self.ao_of_type[lb], self.ao_of_type[ld]
+                if ab is None or ad is None:
+                    continue
+                out["spd"[lb] + "spd"[ld]] = float(Dc[ab][:, ad].abs().max().item())
+        return out
+
+    def run_K(self, Dc, classes=None, maxD=None):
+        t = self.torch
+        L = lib()
+        maxD = maxD or self.max_D(Dc)
+        out = {}
+        for cls in (classes or P.k_classes()):
+            a, b, c, d = cls
+            ac, ab, cd, bd = a + c, a + b, c + d, b + d
+            if ac not in self.ik or ab not in self.kc or cd not in self.kc:
+                continue
+            ik = self.ik[ac]
+            if ac not in out:
+                out[ac] = t.zeros(ik["count"], dtype=t.float64, device=Dc.device)
+            args = [out[ac].data_ptr(), ik["i"].data_ptr(), ik["k"].data_ptr(), ik["count"]]
+            for x in "spd":
+                if x in cls:
+                    l = P.LVAL[x]
+                    args += [self.prim_info[l].data_ptr(), self.prim_aoinds[l].data_ptr(), self.prim_count[l]]
+            args += [maxD[bd], Dc.data_ptr(), self.naos]
+            A, Cc = self.kc[ab], self.kc[cd]
+            for nm in ("Q", "D_inds", "displs", "counts", "pair_data"):
+                args.append(A[nm].data_ptr())
+                if cd != ab:
+                    args.append(Cc[nm].data_ptr())
+            args += [self.dev.boys_table.data_ptr(), self.dev.boys_ft.data_ptr(), self.omega, self.thr]
+            fn = getattr(L, "vlxmi_launch_computeExchangeFock" + cls.upper())
+            check(fn(*args, self.dev.stream_ptr()), "computeExchangeFock" + cls.upper())
+        return out
+
+    def build(self, D_sph, want_k=True):
+        """J and K in the output basis for density D (device tensors)."""
+        Dc = self.cart_density(D_sph)
+        mJ = self.run_J(Dc)
+        Jc = self.assemble_J(mJ)
+        J = self.T @ Jc @ self.T.t()
+        K = None
+        if want_k:
+            mK = self.run_K(Dc)
+            Kc = self.assemble_K(mK)
+            K = self.T @ Kc @ self.T.t()
+        return J, K
+
+    def energies(self, D_sph, J, K):
+        t = self.torch
+        if not t.is_tensor(D_sph):
+            D_sph = self.dev.tensor(np.asarray(D_sph, dtype=np.float64))
+        EJ = 0.5 * float((D_sph * J).sum().item())
+        EK = 0.5 * float((D_sph * K).sum().item()) if K is not None else None
+        return EJ, EK

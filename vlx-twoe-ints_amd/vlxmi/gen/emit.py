@@ -1,0 +1,544 @@
+"""HIP (gfx950) source emitter for the J, K and Schwarz kernels.
+
+Kernel names and parameter lists are the reference's:
+  computeCoulombFock{AB}{CD}    gen_coulomb.py:74-93
+  computeExchangeFock{A}{B}{C}{D}  gen_exchange.py:75-115 (the cd != ab arguments
+                                   are only present when the two pair types differ,
+                                   gen_exchange.py:93-110)
+The bodies are this project's own design (see the module docstrings of
+``vlxmi.pairs`` for the list-order contract they rely on):
+
+J: one lane owns one bra primitive-pair block (all na*nb components); the
+   wave walks the ket blocks in order, staging per-block records (eta, Q, QC,
+   CD, S*D*w per component, |Q*D| per component, block bound) in LDS, one record
+   per lane; each lane evaluates the whole primitive quartet by straight-line
+   HGP code and digests every component with the reference's per-component
+   screen |Q_ij Q_kl D_kl| > thr (as D_kl-weight select against the lane's
+   per-component threshold thr/|Q_ij|).  Lanes leave the scan at the first ket
+   block whose bound fails (blocks are sorted).  grid.y splits the ket list;
+   with more than one split the per-lane sums are added atomically.
+K: one workgroup (one wave) owns one (I,K) primitive block of the ik list;
+   lanes take partner blocks J of I, the wave walks the partner blocks L of K in
+   order (LDS-staged records), each lane evaluates (IJ|KL) and digests with the
+   reference's per-component screen |Q_ij Q_kl maxD_bd| > thr against D_AO
+   gathers; the per-lane K partial sums are combined by a wavefront xor-shuffle
+   reduction and lane 0 adds them to mat_K.
+Schwarz: one lane per primitive-pair block computes the (ab|ab) diagonal.
+"""
+
+from __future__ import annotations
+
+from .osdag import QuartetDAG, tri
+
+LV = {"s": 0, "p": 1, "d": 2}
+NC = {"s": 1, "p": 3, "d": 6}
+
+KET_TILE = 64
+
+
+def _types_in(cls):
+    return [x for x in "spd" if x in cls]
+
+
+# --------------------------------------------------------------------------
+# common pieces
+# --------------------------------------------------------------------------
+
+def quartet_prologue(L, ty_geo="double", omega=False, ind="    "):
+    """Quartet prefactors.  Expects in scope: zeta, inv_S1, P_x.., PA_x.., AB_x..
+    (bra), eta, inv_eta, Q_x.., QC_x.., CD_x.. (ket), boys tables."""
+    o = []
+    o.append("const double S4 = zeta + eta;")
+    o.append("const double inv_S4 = 1.0 / S4;")
+    o.append("const double rho = zeta * eta * inv_S4;")
+    o.append("const double PQ_x = Q_x - P_x, PQ_y = Q_y - P_y, PQ_z = Q_z - P_z;")
+    o.append("double Tb = rho * (PQ_x * PQ_x + PQ_y * PQ_y + PQ_z * PQ_z);")
+    o.append("double Lambda = sqrt(4.0 * rho * MATH_CONST_INV_PI);")
+    if omega:
+        o.append("double d2 = 1.0;")
+        o.append("if (omega != 0.0) { d2 = omega * omega / (rho + omega * omega); Tb *= d2; Lambda *= sqrt(d2); }")
+    o.append("double Fb[%d];" % (L + 1))
+    o.append("vlxmi::boys<%d>(Tb, boys_func_table, Fb);" % L)
+    if omega and L > 0:
+        o.append("if (omega != 0.0) { double dp = d2; "
+                 + " ".join("Fb[%d] *= dp; dp *= d2;" % m for m in range(1, L + 1)) + " }")
+    for m in range(L + 1):
+        o.append("const %s F%d = %s(Fb[%d] * Lambda);" % (ty_geo, m, ty_geo, m))
+    o.append("const %s WP_x = %s(eta * inv_S4 * PQ_x), WP_y = %s(eta * inv_S4 * PQ_y), WP_z = %s(eta * inv_S4 * PQ_z);"
+             % (ty_geo, ty_geo, ty_geo, ty_geo))
+    o.append("const %s WQ_x = %s(-zeta * inv_S4 * PQ_x), WQ_y = %s(-zeta * inv_S4 * PQ_y), WQ_z = %s(-zeta * inv_S4 * PQ_z);"
+             % (ty_geo, ty_geo, ty_geo, ty_geo))
+    o.append("const %s rz = %s(eta * inv_S4), re = %s(zeta * inv_S4);" % (ty_geo, ty_geo, ty_geo))
+    o.append("const %s h1 = %s(0.5 * inv_S1), h2 = %s(0.5 * inv_eta), h4 = %s(0.5 * inv_S4);"
+             % (ty_geo, ty_geo, ty_geo, ty_geo))
+    o.append("const %s gPA_x = %s(PA_x), gPA_y = %s(PA_y), gPA_z = %s(PA_z);" % ((ty_geo,) * 4))
+    o.append("const %s gQC_x = %s(QC_x), gQC_y = %s(QC_y), gQC_z = %s(QC_z);" % ((ty_geo,) * 4))
+    o.append("const %s gAB_x = %s(AB_x), gAB_y = %s(AB_y), gAB_z = %s(AB_z);" % ((ty_geo,) * 4))
+    o.append("const %s gCD_x = %s(CD_x), gCD_y = %s(CD_y), gCD_z = %s(CD_z);" % ((ty_geo,) * 4))
+    return [ind + x for x in o]
+
+
+def _dag_lines(dag):
+    # the DAG refers to PA_/QC_/AB_/CD_ of the geometric type: rename to g*
+    out = []
+    for ln in dag.mult_decls() + dag.lines:
+        ln = ln.replace("PA_", "gPA_").replace("QC_", "gQC_").replace("AB_", "gAB_").replace("CD_", "gCD_")
+        out.append(ln)
+    return out
+
+
+def j_signature(cls):
+    a, b, c, d = cls
+    ab, cd = a + b, c + d
+    p = ["double* __restrict__ mat_J"]
+    for x in "spd":
+        if x in cls:
+            p.append("const double* __restrict__ %s_prim_info" % x)
+            p.append("const uint32_t %s_prim_count" % x)
+    p += ["const double* __restrict__ %s_mat_D" % cd,
+          "const double* __restrict__ %s_mat_Q_local" % ab,
+          "const double* __restrict__ %s_mat_Q" % cd,
+          "const uint32_t* __restrict__ %s_first_inds_local" % ab,
+          "const uint32_t* __restrict__ %s_second_inds_local" % ab,
+          "const double* __restrict__ %s_pair_data_local" % ab,
+          "const uint32_t %s_prim_pair_count_local" % ab,
+          "const uint32_t* __restrict__ %s_first_inds" % cd,
+          "const uint32_t* __restrict__ %s_second_inds" % cd,
+          "const double* __restrict__ %s_pair_data" % cd,
+          "const uint32_t %s_prim_pair_count" % cd,
+          "const double* __restrict__ boys_func_table",
+          "const double* __restrict__ boys_func_ft",
+          "const double eri_threshold"]
+    return p
+
+
+def k_signature(cls):
+    a, b, c, d = cls
+    ac, ab, cd, bd = a + c, a + b, c + d, b + d
+    p = ["double* __restrict__ mat_K",
+         "const uint32_t* __restrict__ pair_inds_i_for_K_%s" % ac,
+         "const uint32_t* __restrict__ pair_inds_k_for_K_%s" % ac,
+         "const uint32_t pair_inds_count_for_K_%s" % ac]
+    for x in "spd":
+        if x in cls:
+            p.append("const double* __restrict__ %s_prim_info" % x)
+            p.append("const uint32_t* __restrict__ %s_prim_aoinds" % x)
+            p.append("const uint32_t %s_prim_count" % x)
+    p += ["const double %s_max_D" % bd,
+          "const double* __restrict__ mat_D_full_AO",
+          "const uint32_t naos"]
+    for nm, ty in (("Q_K", "const double*"), ("D_inds_K", "const uint32_t*"),
+                   ("pair_displs_K", "const uint32_t*"), ("pair_counts_K", "const uint32_t*"),
+                   ("pair_data_K", "const double*")):
+        p.append("%s __restrict__ %s_%s" % (ty, nm, ab))
+        if cd != ab:
+            p.append("%s __restrict__ %s_%s" % (ty, nm, cd))
+    p += ["const double* __restrict__ boys_func_table",
+          "const double* __restrict__ boys_func_ft",
+          "const double omega",
+          "const double eri_threshold"]
+    return p
+
+
+def _argnames(sig):
+    return [s.split()[-1] for s in sig]
+
+
+def _strip_restrict(sig):
+    return [s.replace(" __restrict__", "") for s in sig]
+
+
+# --------------------------------------------------------------------------
+# J kernel
+# --------------------------------------------------------------------------
+
+def emit_j(cls, mixed=False):
+    """Source of computeCoulombFock{CLS} and its C-ABI launcher."""
+    a, b, c, d = cls
+    la, lb, lc, ld = LV[a], LV[b], LV[c], LV[d]
+    na, nb, nc, nd = NC[a], NC[b], NC[c], NC[d]
+    ab, cd = a + b, c + d
+    nab, ncd = na * nb, nc * nd
+    L = la + lb + lc + ld
+    rec = 12 + 2 * ncd
+    rec += rec % 2
+    name = "computeCoulombFock" + cls.upper()
+    sig = j_signature(cls)
+    ty = "double"
+    o = []
+    o.append("__global__ void __launch_bounds__(TILE_SIZE_J)")
+    o.append("%s(%s)" % (name, ",\n    ".join(sig)))
+    o.append("{")
+    o.append("    constexpr int NA = %d, NB = %d, NC = %d, ND = %d, NAB = %d, NCD = %d;" % (na, nb, nc, nd, nab, ncd))
+    o.append("    constexpr int KT = %d, REC = %d;" % (KET_TILE, rec))
+    o.append("    __shared__ double ket[KT * REC];")
+    o.append("    const uint32_t lane = threadIdx.x;")
+    o.append("    const uint32_t nbra = %s_prim_pair_count_local / NAB;" % ab)
+    o.append("    const uint32_t nket = %s_prim_pair_count / NCD;" % cd)
+    o.append("    const uint32_t bb = blockIdx.x * TILE_SIZE_J + lane;")
+    o.append("    const uint32_t chunk = (nket + gridDim.y - 1) / gridDim.y;")
+    o.append("    const uint32_t k_begin = blockIdx.y * chunk;")
+    o.append("    const uint32_t k_end = (k_begin + chunk < nket) ? (k_begin + chunk) : nket;")
+    o.append("    const double thr = eri_threshold;")
+    o.append("    bool active = bb < nbra;")
+    o.append("    double zeta = 1.0, inv_S1 = 1.0, P_x = 0.0, P_y = 0.0, P_z = 0.0;")
+    o.append("    double PA_x = 0.0, PA_y = 0.0, PA_z = 0.0, AB_x = 0.0, AB_y = 0.0, AB_z = 0.0;")
+    o.append("    double qmax = 0.0;")
+    o.append("    double tau[NAB];")
+    o.append("    double acc[NAB];")
+    o.append("    #pragma unroll")
+    o.append("    for (int t = 0; t < NAB; ++t) { acc[t] = 0.0; tau[t] = 0.0; }")
+    o.append("    if (active) {")
+    o.append("        const uint32_t e0 = bb * NAB;")
+    o.append("        const uint32_t I = %s_first_inds_local[e0] / NA;" % ab)
+    o.append("        const uint32_t J = %s_second_inds_local[e0] / NB;" % ab)
+    o.append("        const double ai = %s_prim_info[I];" % a)
+    o.append("        const double A_x = %s_prim_info[I + %s_prim_count * 2], A_y = %s_prim_info[I + %s_prim_count * 3], A_z = %s_prim_info[I + %s_prim_count * 4];" % ((a,) * 6))
+    o.append("        const double aj = %s_prim_info[J];" % b)
+    o.append("        const double B_x = %s_prim_info[J + %s_prim_count * 2], B_y = %s_prim_info[J + %s_prim_count * 3], B_z = %s_prim_info[J + %s_prim_count * 4];" % ((b,) * 6))
+    o.append("        zeta = ai + aj; inv_S1 = 1.0 / zeta;")
+    o.append("        P_x = (ai * A_x + aj * B_x) * inv_S1; P_y = (ai * A_y + aj * B_y) * inv_S1; P_z = (ai * A_z + aj * B_z) * inv_S1;")
+    o.append("        PA_x = (aj * inv_S1) * (B_x - A_x); PA_y = (aj * inv_S1) * (B_y - A_y); PA_z = (aj * inv_S1) * (B_z - A_z);")
+    o.append("        AB_x = A_x - B_x; AB_y = A_y - B_y; AB_z = A_z - B_z;")
+    o.append("        #pragma unroll")
+    o.append("        for (int t = 0; t < NAB; ++t) {")
+    o.append("            const double q = fabs(%s_mat_Q_local[e0 + t]);" % ab)
+    o.append("            qmax = fmax(qmax, q);")
+    o.append("            tau[t] = thr / q;")
+    o.append("        }")
+    o.append("    }")
+    o.append("    for (uint32_t t0 = k_begin; t0 < k_end; t0 += KT) {")
+    o.append("        if (!__any(active)) break;")
+    o.append("        __syncthreads();")
+    o.append("        {")
+    o.append("            const uint32_t kb = t0 + lane;")
+    o.append("            double* R = ket + lane * REC;")
+    o.append("            if (lane < KT && kb < k_end) {")
+    o.append("                const uint32_t f0 = kb * NCD;")
+    o.append("                const uint32_t k = %s_first_inds[f0] / NC;" % cd)
+    o.append("                const uint32_t l = %s_second_inds[f0] / ND;" % cd)
+    o.append("                const double ak = %s_prim_info[k];" % c)
+    o.append("                const double C_x = %s_prim_info[k + %s_prim_count * 2], C_y = %s_prim_info[k + %s_prim_count * 3], C_z = %s_prim_info[k + %s_prim_count * 4];" % ((c,) * 6))
+    o.append("                const double al = %s_prim_info[l];" % d)
+    o.append("                const double D_x = %s_prim_info[l + %s_prim_count * 2], D_y = %s_prim_info[l + %s_prim_count * 3], D_z = %s_prim_info[l + %s_prim_count * 4];" % ((d,) * 6))
+    o.append("                const double et = ak + al, iet = 1.0 / et;")
+    o.append("                R[1] = et; R[2] = iet;")
+    o.append("                R[3] = (ak * C_x + al * D_x) * iet; R[4] = (ak * C_y + al * D_y) * iet; R[5] = (ak * C_z + al * D_z) * iet;")
+    o.append("                R[6] = (al * iet) * (D_x - C_x); R[7] = (al * iet) * (D_y - C_y); R[8] = (al * iet) * (D_z - C_z);")
+    o.append("                R[9] = C_x - D_x; R[10] = C_y - D_y; R[11] = C_z - D_z;")
+    o.append("                double beta = 0.0;")
+    o.append("                #pragma unroll")
+    o.append("                for (int t = 0; t < NCD; ++t) {")
+    o.append("                    const double Dkl = %s_mat_D[f0 + t];" % cd)
+    if c == d:
+        o.append("                    const double w = (%s_first_inds[f0 + t] != %s_second_inds[f0 + t]) ? 2.0 : 1.0;" % (cd, cd))
+    else:
+        o.append("                    const double w = 2.0;")
+    o.append("                    R[12 + t] = %s_pair_data[f0 + t] * Dkl * w;" % cd)
+    o.append("                    const double qd = fabs(%s_mat_Q[f0 + t] * Dkl);" % cd)
+    o.append("                    R[12 + NCD + t] = qd;")
+    o.append("                    beta = fmax(beta, qd);")
+    o.append("                }")
+    o.append("                R[0] = beta;")
+    o.append("            }")
+    o.append("        }")
+    o.append("        __syncthreads();")
+    o.append("        const uint32_t nt = (k_end - t0 < (uint32_t)KT) ? (k_end - t0) : (uint32_t)KT;")
+    o.append("        for (uint32_t s = 0; s < nt; ++s) {")
+    o.append("            const double* R = ket + s * REC;")
+    o.append("            if (active && qmax * R[0] * (1.0 + 1e-13) <= thr) active = false;")
+    o.append("            if (!__any(active)) break;")
+    o.append("            if (active) {")
+    o.append("                const double eta = R[1], inv_eta = R[2];")
+    o.append("                const double Q_x = R[3], Q_y = R[4], Q_z = R[5];")
+    o.append("                const double QC_x = R[6], QC_y = R[7], QC_z = R[8];")
+    o.append("                const double CD_x = R[9], CD_y = R[10], CD_z = R[11];")
+    body = quartet_prologue(L, ty_geo=ty, ind="                ")
+    o += body
+    dag = QuartetDAG(la, lb, lc, ld, ty=ty, rule_vrr="first", rule_hrr="first")
+    dig = []
+    for ic, cc in enumerate(tri(lc)):
+        for idd, dd in enumerate(tri(ld)):
+            tcd = ic * nd + idd
+            dig.append("const double Dp%d = R[%d], Qd%d = R[%d];" % (tcd, 12 + tcd, tcd, 12 + ncd + tcd))
+            for ia, aa in enumerate(tri(la)):
+                for ib, bbb in enumerate(tri(lb)):
+                    tab = ia * nb + ib
+                    x = dag.X(aa, bbb, cc, dd)
+                    dag.lines.append("acc[%d] += (Qd%d > tau[%d]) ? (%s * Dp%d) : 0.0;" % (tab, tcd, tab, x, tcd))
+    # the digestion loads must precede their uses: emit loads first
+    o += ["                " + x for x in dig]
+    o += ["                " + x for x in _dag_lines(dag)]
+    o.append("            }")
+    o.append("        }")
+    o.append("    }")
+    o.append("    if (bb < nbra) {")
+    o.append("        const uint32_t e0 = bb * NAB;")
+    o.append("        #pragma unroll")
+    o.append("        for (int t = 0; t < NAB; ++t) {")
+    o.append("            const double v = %s_pair_data_local[e0 + t] * acc[t];" % ab)
+    o.append("            if (gridDim.y > 1) atomicAdd(&mat_J[e0 + t], v); else mat_J[e0 + t] += v;")
+    o.append("        }")
+    o.append("    }")
+    o.append("}")
+    o.append("")
+    # launcher
+    lsig = _strip_restrict(sig)
+    args = _argnames(sig)
+    o.append('extern "C" int vlxmi_launch_%s(%s,\n    uint32_t ket_split, void* stream)' % (name, ",\n    ".join(lsig)))
+    o.append("{")
+    o.append("    if (%s_prim_pair_count_local %% %d != 0 || %s_prim_pair_count %% %d != 0) return VLXMI_ERR_LAYOUT;" % (ab, nab, cd, ncd))
+    o.append("    const uint32_t nbra = %s_prim_pair_count_local / %d, nket = %s_prim_pair_count / %d;" % (ab, nab, cd, ncd))
+    o.append("    if (nbra == 0 || nket == 0) return VLXMI_OK;")
+    o.append("    if (!mat_J || !boys_func_table) return VLXMI_ERR_ARG;")
+    o.append("    if (ket_split == 0) ket_split = 1;")
+    o.append("    if (ket_split > nket) ket_split = nket;")
+    o.append("    dim3 grid((nbra + TILE_SIZE_J - 1) / TILE_SIZE_J, ket_split);")
+    o.append("    hipLaunchKernelGGL(%s, grid, dim3(TILE_SIZE_J), 0, (hipStream_t)stream, %s);" % (name, ", ".join(args)))
+    o.append("    return (int)hipGetLastError();")
+    o.append("}")
+    return "\n".join(o) + "\n"
+
+
+# --------------------------------------------------------------------------
+# K kernel
+# --------------------------------------------------------------------------
+
+def emit_k(cls):
+    a, b, c, d = cls
+    la, lb, lc, ld = LV[a], LV[b], LV[c], LV[d]
+    na, nb, nc, nd = NC[a], NC[b], NC[c], NC[d]
+    ac, ab, cd, bd = a + c, a + b, c + d, b + d
+    nac, nab, ncd = na * nc, na * nb, nc * nd
+    L = la + lb + lc + ld
+    rec = 12 + 2 * ncd
+    rec += rec % 2
+    name = "computeExchangeFock" + cls.upper()
+    sig = k_signature(cls)
+    ty = "double"
+    o = []
+    o.append("__global__ void __launch_bounds__(TILE_SIZE_K)")
+    o.append("%s(%s)" % (name, ",\n    ".join(sig)))
+    o.append("{")
+    o.append("    constexpr int NA = %d, NB = %d, NC = %d, ND = %d, NAC = %d, NAB = %d, NCD = %d;" % (na, nb, nc, nd, nac, nab, ncd))
+    o.append("    constexpr int KT = %d, REC = %d;" % (KET_TILE, rec))
+    o.append("    __shared__ double ket[KT * REC];")
+    o.append("    __shared__ uint32_t ket_cg[KT * ND];")
+    o.append("    const uint32_t lane = threadIdx.x;")
+    o.append("    const uint32_t blk = blockIdx.x;")
+    o.append("    if ((blk + 1) * NAC > pair_inds_count_for_K_%s) return;" % ac)
+    o.append("    const double maxD = %s_max_D;" % bd)
+    o.append("    if (!(maxD > 0.0)) return;")
+    o.append("    const double thr = eri_threshold;")
+    o.append("    const double thr_eff = thr / maxD;")
+    o.append("    const uint32_t e0 = blk * NAC;")
+    o.append("    const uint32_t I = pair_inds_i_for_K_%s[e0] / NA;" % ac)
+    o.append("    const uint32_t K = pair_inds_k_for_K_%s[e0] / NC;" % ac)
+    o.append("    const double ai = %s_prim_info[I];" % a)
+    o.append("    const double A_x = %s_prim_info[I + %s_prim_count * 2], A_y = %s_prim_info[I + %s_prim_count * 3], A_z = %s_prim_info[I + %s_prim_count * 4];" % ((a,) * 6))
+    o.append("    const double ak = %s_prim_info[K];" % c)
+    o.append("    const double C_x = %s_prim_info[K + %s_prim_count * 2], C_y = %s_prim_info[K + %s_prim_count * 3], C_z = %s_prim_info[K + %s_prim_count * 4];" % ((c,) * 6))
+    o.append("    uint32_t displ_i[NA], displ_k[NC];")
+    o.append("    #pragma unroll")
+    o.append("    for (int t = 0; t < NA; ++t) displ_i[t] = pair_displs_K_%s[I * NA + t];" % ab)
+    o.append("    #pragma unroll")
+    o.append("    for (int t = 0; t < NC; ++t) displ_k[t] = pair_displs_K_%s[K * NC + t];" % cd)
+    o.append("    const uint32_t nJ = pair_counts_K_%s[I * NA] / NB;" % ab)
+    o.append("    const uint32_t nL = pair_counts_K_%s[K * NC] / ND;" % cd)
+    o.append("    double acc[NAC];")
+    o.append("    #pragma unroll")
+    o.append("    for (int t = 0; t < NAC; ++t) acc[t] = 0.0;")
+    o.append("    for (uint32_t jc = 0; jc < nJ; jc += TILE_SIZE_K) {")
+    o.append("        const uint32_t jp = jc + lane;")
+    o.append("        bool active = jp < nJ;")
+    o.append("        double zeta = 1.0, inv_S1 = 1.0, P_x = 0.0, P_y = 0.0, P_z = 0.0;")
+    o.append("        double PA_x = 0.0, PA_y = 0.0, PA_z = 0.0, AB_x = 0.0, AB_y = 0.0, AB_z = 0.0;")
+    o.append("        double qij = 0.0;")
+    o.append("        double tau[NAB], Sab[NAB];")
+    o.append("        uint32_t cgJ[NB];")
+    o.append("        #pragma unroll")
+    o.append("        for (int t = 0; t < NAB; ++t) { tau[t] = 0.0; Sab[t] = 0.0; }")
+    o.append("        #pragma unroll")
+    o.append("        for (int t = 0; t < NB; ++t) cgJ[t] = 0;")
+    o.append("        if (active) {")
+    o.append("            const uint32_t jcomb = D_inds_K_%s[displ_i[0] + jp * NB];" % ab)
+    o.append("            const uint32_t J = jcomb / NB;")
+    o.append("            const double aj = %s_prim_info[J];" % b)
+    o.append("            const double B_x = %s_prim_info[J + %s_prim_count * 2], B_y = %s_prim_info[J + %s_prim_count * 3], B_z = %s_prim_info[J + %s_prim_count * 4];" % ((b,) * 6))
+    o.append("            zeta = ai + aj; inv_S1 = 1.0 / zeta;")
+    o.append("            P_x = (ai * A_x + aj * B_x) * inv_S1; P_y = (ai * A_y + aj * B_y) * inv_S1; P_z = (ai * A_z + aj * B_z) * inv_S1;")
+    o.append("            PA_x = (aj * inv_S1) * (B_x - A_x); PA_y = (aj * inv_S1) * (B_y - A_y); PA_z = (aj * inv_S1) * (B_z - A_z);")
+    o.append("            AB_x = A_x - B_x; AB_y = A_y - B_y; AB_z = A_z - B_z;")
+    o.append("            #pragma unroll")
+    o.append("            for (int ca = 0; ca < NA; ++ca) {")
+    o.append("                #pragma unroll")
+    o.append("                for (int cb = 0; cb < NB; ++cb) {")
+    o.append("                    const uint32_t pos = displ_i[ca] + jp * NB + cb;")
+    o.append("                    const double q = fabs(Q_K_%s[pos]);" % ab)
+    o.append("                    qij = fmax(qij, q);")
+    o.append("                    tau[ca * NB + cb] = thr_eff / q;")
+    o.append("                    Sab[ca * NB + cb] = pair_data_K_%s[pos];" % ab)
+    o.append("                }")
+    o.append("            }")
+    o.append("            #pragma unroll")
+    o.append("            for (int cb = 0; cb < NB; ++cb) {")
+    o.append("                const uint32_t jj = D_inds_K_%s[displ_i[0] + jp * NB + cb];" % ab)
+    if b == "s":
+        o.append("                cgJ[cb] = %s_prim_aoinds[jj];" % b)
+    else:
+        o.append("                cgJ[cb] = %s_prim_aoinds[(jj / NB) + %s_prim_count * (jj %% NB)];" % (b, b))
+    o.append("            }")
+    o.append("        }")
+    o.append("        for (uint32_t t0 = 0; t0 < nL; t0 += KT) {")
+    o.append("            if (!__any(active)) break;")
+    o.append("            __syncthreads();")
+    o.append("            {")
+    o.append("                const uint32_t lp = t0 + lane;")
+    o.append("                double* R = ket + lane * REC;")
+    o.append("                if (lane < KT && lp < nL) {")
+    o.append("                    const uint32_t lcomb = D_inds_K_%s[displ_k[0] + lp * ND];" % cd)
+    o.append("                    const uint32_t Lp = lcomb / ND;")
+    o.append("                    const double al = %s_prim_info[Lp];" % d)
+    o.append("                    const double D_x = %s_prim_info[Lp + %s_prim_count * 2], D_y = %s_prim_info[Lp + %s_prim_count * 3], D_z = %s_prim_info[Lp + %s_prim_count * 4];" % ((d,) * 6))
+    o.append("                    const double et = ak + al, iet = 1.0 / et;")
+    o.append("                    R[1] = et; R[2] = iet;")
+    o.append("                    R[3] = (ak * C_x + al * D_x) * iet; R[4] = (ak * C_y + al * D_y) * iet; R[5] = (ak * C_z + al * D_z) * iet;")
+    o.append("                    R[6] = (al * iet) * (D_x - C_x); R[7] = (al * iet) * (D_y - C_y); R[8] = (al * iet) * (D_z - C_z);")
+    o.append("                    R[9] = C_x - D_x; R[10] = C_y - D_y; R[11] = C_z - D_z;")
+    o.append("                    double qk = 0.0;")
+    o.append("                    #pragma unroll")
+    o.append("                    for (int cc = 0; cc < NC; ++cc) {")
+    o.append("                        #pragma unroll")
+    o.append("                        for (int cdd = 0; cdd < ND; ++cdd) {")
+    o.append("                            const uint32_t pos = displ_k[cc] + lp * ND + cdd;")
+    o.append("                            const double q = fabs(Q_K_%s[pos]);" % cd)
+    o.append("                            qk = fmax(qk, q);")
+    o.append("                            R[12 + cc * ND + cdd] = pair_data_K_%s[pos];" % cd)
+    o.append("                            R[12 + NCD + cc * ND + cdd] = q;")
+    o.append("                        }")
+    o.append("                    }")
+    o.append("                    R[0] = qk;")
+    o.append("                    #pragma unroll")
+    o.append("                    for (int cdd = 0; cdd < ND; ++cdd) {")
+    o.append("                        const uint32_t ll = D_inds_K_%s[displ_k[0] + lp * ND + cdd];" % cd)
+    if d == "s":
+        o.append("                        ket_cg[lane * ND + cdd] = %s_prim_aoinds[ll];" % d)
+    else:
+        o.append("                        ket_cg[lane * ND + cdd] = %s_prim_aoinds[(ll / ND) + %s_prim_count * (ll %% ND)];" % (d, d))
+    o.append("                    }")
+    o.append("                }")
+    o.append("            }")
+    o.append("            __syncthreads();")
+    o.append("            const uint32_t nt = (nL - t0 < (uint32_t)KT) ? (nL - t0) : (uint32_t)KT;")
+    o.append("            for (uint32_t s = 0; s < nt; ++s) {")
+    o.append("                const double* R = ket + s * REC;")
+    o.append("                if (active && qij * R[0] * maxD * (1.0 + 1e-13) <= thr) active = false;")
+    o.append("                if (!__any(active)) break;")
+    o.append("                if (active) {")
+    o.append("                    const double eta = R[1], inv_eta = R[2];")
+    o.append("                    const double Q_x = R[3], Q_y = R[4], Q_z = R[5];")
+    o.append("                    const double QC_x = R[6], QC_y = R[7], QC_z = R[8];")
+    o.append("                    const double CD_x = R[9], CD_y = R[10], CD_z = R[11];")
+    o += quartet_prologue(L, ty_geo=ty, omega=True, ind="                    ")
+    pre = []
+    for ibb in range(nb):
+        for idd in range(nd):
+            pre.append("const double Dj%d_%d = mat_D_full_AO[(size_t)cgJ[%d] * naos + ket_cg[s * ND + %d]];" % (ibb, idd, ibb, idd))
+    for icc in range(nc):
+        for idd in range(nd):
+            tcd = icc * nd + idd
+            pre.append("const double Sc%d = R[%d], Qk%d = R[%d];" % (tcd, 12 + tcd, tcd, 12 + ncd + tcd))
+    dag = QuartetDAG(la, lb, lc, ld, ty=ty, rule_vrr="first", rule_hrr="first")
+    # digestion: acc[a,c] += Sab * sum_{b,d} mask * X * Sc * D
+    for ic, cc in enumerate(tri(lc)):
+        for ia, aa in enumerate(tri(la)):
+            tac = ia * nc + ic
+            for ib, bbx in enumerate(tri(lb)):
+                tab = ia * nb + ib
+                terms = []
+                for idd, dd in enumerate(tri(ld)):
+                    tcd = ic * nd + idd
+                    x = dag.X(aa, bbx, cc, dd)
+                    terms.append("((Qk%d > tau[%d]) ? (%s * Sc%d * Dj%d_%d) : 0.0)" % (tcd, tab, x, tcd, ib, idd))
+                dag.lines.append("acc[%d] += Sab[%d] * (%s);" % (tac, tab, " + ".join(terms)))
+    o += ["                    " + x for x in pre]
+    o += ["                    " + x for x in _dag_lines(dag)]
+    o.append("                }")
+    o.append("            }")
+    o.append("        }")
+    o.append("    }")
+    o.append("    #pragma unroll")
+    o.append("    for (int t = 0; t < NAC; ++t) {")
+    o.append("        double v = acc[t];")
+    o.append("        #pragma unroll")
+    o.append("        for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);")
+    o.append("        if (lane == 0) mat_K[e0 + t] += v;")
+    o.append("    }")
+    o.append("}")
+    o.append("")
+    lsig = _strip_restrict(sig)
+    args = _argnames(sig)
+    o.append('extern "C" int vlxmi_launch_%s(%s,\n    void* stream)' % (name, ",\n    ".join(lsig)))
+    o.append("{")
+    o.append("    if (pair_inds_count_for_K_%s %% %d != 0) return VLXMI_ERR_LAYOUT;" % (ac, nac))
+    o.append("    const uint32_t nblk = pair_inds_count_for_K_%s / %d;" % (ac, nac))
+    o.append("    if (nblk == 0) return VLXMI_OK;")
+    o.append("    if (!mat_K || !mat_D_full_AO || !boys_func_table) return VLXMI_ERR_ARG;")
+    o.append("    hipLaunchKernelGGL(%s, dim3(nblk), dim3(TILE_SIZE_K), 0, (hipStream_t)stream, %s);" % (name, ", ".join(args)))
+    o.append("    return (int)hipGetLastError();")
+    o.append("}")
+    return "\n".join(o) + "\n"
+
+
+# --------------------------------------------------------------------------
+# Schwarz kernel
+# --------------------------------------------------------------------------
+
+def emit_schwarz(pc):
+    a, b = pc
+    la, lb = LV[a], LV[b]
+    na, nb = NC[a], NC[b]
+    nab = na * nb
+    L = 2 * (la + lb)
+    name = "vlxmi_schwarz_%s" % pc
+    o = []
+    o.append("__global__ void __launch_bounds__(64)")
+    o.append("%s_kernel(const double* __restrict__ a_info, const uint32_t a_count, const double* __restrict__ b_info, const uint32_t b_count," % name)
+    o.append("    const uint32_t* __restrict__ Iblk, const uint32_t* __restrict__ Jblk, const double* __restrict__ S, const uint32_t nblk,")
+    o.append("    double* __restrict__ Qout, const double* __restrict__ boys_func_table)")
+    o.append("{")
+    o.append("    const uint32_t t = blockIdx.x * 64 + threadIdx.x;")
+    o.append("    if (t >= nblk) return;")
+    o.append("    const uint32_t I = Iblk[t], J = Jblk[t];")
+    o.append("    const double ai = a_info[I], aj = b_info[J];")
+    o.append("    const double A_x = a_info[I + a_count * 2], A_y = a_info[I + a_count * 3], A_z = a_info[I + a_count * 4];")
+    o.append("    const double B_x = b_info[J + b_count * 2], B_y = b_info[J + b_count * 3], B_z = b_info[J + b_count * 4];")
+    o.append("    const double zeta = ai + aj, inv_S1 = 1.0 / zeta;")
+    o.append("    const double P_x = (ai * A_x + aj * B_x) * inv_S1, P_y = (ai * A_y + aj * B_y) * inv_S1, P_z = (ai * A_z + aj * B_z) * inv_S1;")
+    o.append("    const double PA_x = (aj * inv_S1) * (B_x - A_x), PA_y = (aj * inv_S1) * (B_y - A_y), PA_z = (aj * inv_S1) * (B_z - A_z);")
+    o.append("    const double AB_x = A_x - B_x, AB_y = A_y - B_y, AB_z = A_z - B_z;")
+    o.append("    const double eta = zeta, inv_eta = inv_S1;")
+    o.append("    const double Q_x = P_x, Q_y = P_y, Q_z = P_z;")
+    o.append("    const double QC_x = PA_x, QC_y = PA_y, QC_z = PA_z;")
+    o.append("    const double CD_x = AB_x, CD_y = AB_y, CD_z = AB_z;")
+    o += quartet_prologue(L, ind="    ")
+    dag = QuartetDAG(la, lb, la, lb, rule_vrr="first", rule_hrr="first")
+    for ia, aa in enumerate(tri(la)):
+        for ib, bbx in enumerate(tri(lb)):
+            tab = ia * nb + ib
+            x = dag.X(aa, bbx, aa, bbx)
+            dag.lines.append("{ const double s = S[t * %d + %d]; Qout[t * %d + %d] = sqrt(fabs(%s * s * s)); }" % (nab, tab, nab, tab, x))
+    o += ["    " + x for x in _dag_lines(dag)]
+    o.append("}")
+    o.append("")
+    o.append('extern "C" int %s(const double* a_info, uint32_t a_count, const double* b_info, uint32_t b_count,' % name)
+    o.append("    const uint32_t* Iblk, const uint32_t* Jblk, const double* S, uint32_t nblk, double* Qout,")
+    o.append("    const double* boys_func_table, void* stream)")
+    o.append("{")
+    o.append("    if (nblk == 0) return VLXMI_OK;")
+    o.append("    if (!a_info || !b_info || !Iblk || !Jblk || !S || !Qout || !boys_func_table) return VLXMI_ERR_ARG;")
+    o.append("    hipLaunchKernelGGL(%s_kernel, dim3((nblk + 63) / 64), dim3(64), 0, (hipStream_t)stream," % name)
+    o.append("        a_info, a_count, b_info, b_count, Iblk, Jblk, S, nblk, Qout, boys_func_table);")
+    o.append("    return (int)hipGetLastError();")
+    o.append("}")
+    return "\n".join(o) + "\n"

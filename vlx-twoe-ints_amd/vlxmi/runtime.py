@@ -1,0 +1,139 @@
+"""ctypes binding of libvlxmi.so over PyTorch-ROCm device tensors.
+
+PyTorch provides device memory and streams only; all arithmetic runs in the
+library's HIP kernels.  The library must exist (``vlxmi.gen.build``); there is
+no fallback path: a missing library or a missing GPU raises.
+"""
+
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+from .gen import emit
+from .pairs import PAIR_CLASSES, j_classes, k_classes
+
+LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "_lib", "libvlxmi.so")
+
+_lib = None
+
+
+class VlxmiError(RuntimeError):
+    pass
+
+
+def _ctype_of(decl: str):
+    decl = decl.replace("const ", "").replace("__restrict__", "").strip()
+    ty = decl.rsplit(" ", 1)[0].strip()
+    if ty.endswith("*"):
+        return C.c_void_p
+    return {"uint32_t": C.c_uint32, "double": C.c_double}[ty]
+
+
+def lib():
+    """Load libvlxmi.so and declare every entry point's argument types."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise VlxmiError("libvlxmi.so not built: run __graft_entry__.build() (or python -m vlxmi.gen.build)")
+    L = C.CDLL(LIB_PATH)
+    for ab, cd in j_classes():
+        cls = (ab + cd).upper()
+        fn = getattr(L, "vlxmi_launch_computeCoulombFock" + cls)
+        fn.argtypes = [_ctype_of(s) for s in emit.j_signature(ab + cd)] + [C.c_uint32, C.c_void_p]
+        fn.restype = C.c_int
+    for cls in k_classes():
+        fn = getattr(L, "vlxmi_launch_computeExchangeFock" + cls.upper())
+        fn.argtypes = [_ctype_of(s) for s in emit.k_signature(cls)] + [C.c_void_p]
+        fn.restype = C.c_int
+    for pc in PAIR_CLASSES:
+        fn = getattr(L, "vlxmi_schwarz_" + pc)
+        fn.argtypes = [C.c_void_p, C.c_uint32, C.c_void_p, C.c_uint32, C.c_void_p, C.c_void_p, C.c_void_p,
+                       C.c_uint32, C.c_void_p, C.c_void_p, C.c_void_p]
+        fn.restype = C.c_int
+    L.vlxmi_boys_table_size.argtypes = [C.POINTER(C.c_uint32)] * 3
+    L.vlxmi_boys_table_fill.argtypes = [C.c_void_p, C.c_void_p]
+    L.vlxmi_boys_eval.argtypes = [C.c_void_p, C.c_uint32, C.c_uint32, C.c_void_p, C.c_void_p, C.c_void_p]
+    L.vlxmi_status_string.restype = C.c_char_p
+    L.vlxmi_status_string.argtypes = [C.c_int]
+    L.vlxmi_version.restype = C.c_int
+    _lib = L
+    return L
+
+
+def check(rc: int, what: str = ""):
+    if rc != 0:
+        msg = lib().vlxmi_status_string(rc).decode()
+        raise VlxmiError("%s failed: %s (%d)" % (what, msg, rc))
+
+
+def exported_symbols():
+    """Names every C-ABI entry point the header declares."""
+    names = ["vlxmi_version", "vlxmi_boys_table_size", "vlxmi_boys_table_fill", "vlxmi_status_string",
+             "vlxmi_boys_eval"]
+    names += ["vlxmi_launch_computeCoulombFock" + (ab + cd).upper() for ab, cd in j_classes()]
+    names += ["vlxmi_launch_computeExchangeFock" + c.upper() for c in k_classes()]
+    names += ["vlxmi_schwarz_" + pc for pc in PAIR_CLASSES]
+    return names
+
+
+def boys_tables_host():
+    nr, nc, nf = C.c_uint32(), C.c_uint32(), C.c_uint32()
+    check(lib().vlxmi_boys_table_size(C.byref(nr), C.byref(nc), C.byref(nf)), "boys_table_size")
+    table = np.zeros(nr.value * nc.value)
+    ft = np.zeros(nf.value)
+    check(lib().vlxmi_boys_table_fill(table.ctypes.data, ft.ctypes.data), "boys_table_fill")
+    return table, ft
+
+
+class Device:
+    """Device-resident Boys tables and the stream used for launches."""
+
+    def __init__(self, device="cuda:0"):
+        import torch
+        if not torch.cuda.is_available():
+            raise VlxmiError("no HIP device visible: the vlxmi kernels need an MI355X (gfx950)")
+        self.torch = torch
+        self.device = torch.device(device)
+        table, ft = boys_tables_host()
+        self.boys_table = torch.from_numpy(table).to(self.device)
+        self.boys_ft = torch.from_numpy(ft).to(self.device)
+
+    def stream_ptr(self):
+        return self.torch.cuda.current_stream(self.device).cuda_stream
+
+    def tensor(self, a, dtype=None):
+        t = self.torch
+        a = np.ascontiguousarray(a)
+        if dtype is None:
+            dtype = {np.dtype(np.float64): t.float64, np.dtype(np.uint32): t.int32,
+                     np.dtype(np.int64): t.int64, np.dtype(np.int32): t.int32, np.dtype(bool): t.bool}[a.dtype]
+        if a.dtype == np.uint32:
+            a = a.view(np.int32)
+        return t.from_numpy(a).to(self.device, dtype=dtype)
+
+    def boys_eval(self, T: np.ndarray, L: int) -> np.ndarray:
+        t = self.torch
+        Td = self.tensor(np.asarray(T, dtype=np.float64))
+        out = t.zeros(Td.numel() * (L + 1), dtype=t.float64, device=self.device)
+        check(lib().vlxmi_boys_eval(Td.data_ptr(), Td.numel(), L, self.boys_table.data_ptr(), out.data_ptr(),
+                                    self.stream_ptr()), "boys_eval")
+        return out.cpu().numpy().reshape(-1, L + 1)
+
+    def schwarz(self, la, lb, info_a, info_b, I, J, S):
+        """GPU Schwarz factors for candidate pair blocks (host arrays in/out)."""
+        t = self.torch
+        pc = "spd"[la] + "spd"[lb]
+        ia, ib = self.tensor(info_a), self.tensor(info_b)
+        Id, Jd = self.tensor(np.asarray(I, dtype=np.uint32)), self.tensor(np.asarray(J, dtype=np.uint32))
+        Sd = self.tensor(np.ascontiguousarray(S, dtype=np.float64).ravel())
+        Q = t.zeros_like(Sd)
+        n = int(np.asarray(I).size)
+        check(getattr(lib(), "vlxmi_schwarz_" + pc)(ia.data_ptr(), info_a.shape[1], ib.data_ptr(), info_b.shape[1],
+                                                      Id.data_ptr(), Jd.data_ptr(), Sd.data_ptr(), n, Q.data_ptr(),
+                                                      self.boys_table.data_ptr(), self.stream_ptr()),
+              "schwarz_" + pc)
+        return Q.cpu().numpy().reshape(np.shape(S))

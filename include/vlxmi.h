@@ -1,0 +1,40 @@
+/* vlxmi C ABI: hand-written entry points (the per-class J/K launchers and the
+ * Schwarz kernels are declared in vlxmi_kernels.h).  All pointers are device
+ * pointers unless stated; every call returns 0 on success, a negative
+ * vlxmi status (vlxmi_status.h) for argument errors, or a positive hipError_t. */
+#ifndef VLXMI_H
+#define VLXMI_H
+#include <stdint.h>
+#include "vlxmi_kernels.h"
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Library ABI version (1). */
+int vlxmi_version(void);
+
+/* Shape of the Boys tables consumed as boys_func_table / boys_func_ft by every
+ * kernel (the reference passes VeloxChem's tables under these names,
+ * gen_coulomb.py:91-92; their layout is this library's own). */
+int vlxmi_boys_table_size(uint32_t* nrow, uint32_t* ncol, uint32_t* nft);
+
+/* Fills HOST buffers table[nrow*ncol] and ft[nft]; the caller uploads them. */
+int vlxmi_boys_table_fill(double* table, double* ft);
+
+/* out[t*(L+1)+m] = F_m(T[t]), the device Boys function (test probe). */
+int vlxmi_boys_eval(const double* T, uint32_t n, uint32_t L, const double* table, double* out, void* stream);
+
+/* Human-readable text of a status code (static string). */
+const char* vlxmi_status_string(int code);
+
+/* Adds to *out the number of primitive shell quartets of one exchange class that
+ * pass QKmax_IJ * QKmax_KL * maxD_bd > thr (thr_over_maxD = thr / maxD_bd). */
+int vlxmi_count_k_quartets(const uint32_t* ikI, const uint32_t* ikK, uint32_t nblk,
+                           const double* qa, const uint32_t* qa_displ, const uint32_t* qa_cnt,
+                           const double* qc, const uint32_t* qc_displ, const uint32_t* qc_cnt,
+                           double thr_over_maxD, unsigned long long* out, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

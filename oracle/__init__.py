@@ -93,7 +93,7 @@ def _ptr(a):
 
 def coulomb_class(la, lb, lc, ld, prim_info, prim_count, cd_mat_D, ab_mat_Q, cd_mat_Q,
                   ab_first, ab_second, ab_pair_data, cd_first, cd_second, cd_pair_data,
-                  thr, mode="component", bra_blocks=None, mat_J=None):
+                  thr, mode="component", bra_blocks=None, mat_J=None, n_eval=None):
     """mat_J (+=) of one J class over the reference-layout lists.
 
     ``prim_info``/``prim_count``: sequences indexed by angular momentum.
@@ -125,7 +125,8 @@ def coulomb_class(la, lb, lc, ld, prim_info, prim_count, cd_mat_D, ab_mat_Q, cd_
         C.c_uint32(nab),
         _ptr(keep(cd_first, np.uint32)), _ptr(keep(cd_second, np.uint32)), _ptr(keep(cd_pair_data, np.float64)),
         C.c_uint32(np.asarray(cd_first).size),
-        C.c_double(thr), C.c_int(0 if mode == "component" else 1), bptr, C.c_uint32(nb), _ptr(mat_J))
+        C.c_double(thr), C.c_int(0 if mode == "component" else 1), bptr, C.c_uint32(nb), _ptr(mat_J),
+        _ptr(n_eval) if n_eval is not None else None)
     if rc != 0:
         raise ValueError("oracle coulomb_class: list sizes are not whole blocks")
     return mat_J
@@ -134,7 +135,7 @@ def coulomb_class(la, lb, lc, ld, prim_info, prim_count, cd_mat_D, ab_mat_Q, cd_
 def exchange_class(la, lb, lc, ld, ik_i, ik_k, prim_info, prim_aoinds, prim_count, max_D_bd,
                    D_AO, naos, Q_ab, Q_cd, Dinds_ab, Dinds_cd, displs_ab, displs_cd,
                    counts_ab, counts_cd, pd_ab, pd_cd, omega, thr, mode="component",
-                   ik_blocks=None, mat_K=None):
+                   ik_blocks=None, mat_K=None, max_j=0, n_eval=None):
     """mat_K (+=) of one K class over the reference-layout CSR lists."""
     arrs = []
 
@@ -166,7 +167,7 @@ def exchange_class(la, lb, lc, ld, ik_i, ik_k, prim_info, prim_aoinds, prim_coun
         _ptr(keep(counts_ab, np.uint32)), _ptr(keep(counts_cd, np.uint32)),
         _ptr(keep(pd_ab, np.float64)), _ptr(keep(pd_cd, np.float64)),
         C.c_double(omega), C.c_double(thr), C.c_int(0 if mode == "component" else 1),
-        bptr, C.c_uint32(nb), _ptr(mat_K))
+        bptr, C.c_uint32(nb), _ptr(mat_K), C.c_uint32(max_j), _ptr(n_eval) if n_eval is not None else None)
     if rc != 0:
         raise ValueError("oracle exchange_class: ik list size is not whole blocks")
     return mat_K

@@ -298,7 +298,8 @@ static void prim_of(const double* info, uint32_t count, uint32_t prim, double* a
  * J for one class (la lb | lc ld) over the reference-layout lists.  Lists are
  * sequences of complete primitive-pair blocks (na*nb entries, component order
  * a-major).  bra_blocks selects the bra blocks to evaluate (NULL: all).
- * mat_J is accumulated (+=) like the reference kernels.
+ * mat_J is accumulated (+=) like the reference kernels.  *n_eval (if not
+ * NULL) is increased by the number of primitive quartets evaluated.
  */
 int orc_coulomb_class(int la, int lb, int lc, int ld,
                       const double* const* prim_info, const uint32_t* prim_count,
@@ -309,8 +310,9 @@ int orc_coulomb_class(int la, int lb, int lc, int ld,
                       const double* cd_pair_data, uint32_t cd_count,
                       double thr, int mode,
                       const uint32_t* bra_blocks, uint32_t n_bra_blocks,
-                      double* mat_J)
+                      double* mat_J, unsigned long long* n_eval)
 {
+    unsigned long long nev = 0;
     const int na = ncart_of(la), nb = ncart_of(lb), nc = ncart_of(lc), nd = ncart_of(ld);
     const int nab = na * nb, ncd = nc * nd;
     if (ab_count % nab || cd_count % ncd) return -1;
@@ -329,7 +331,7 @@ int orc_coulomb_class(int la, int lb, int lc, int ld,
         beta[kb] = m;
     }
 
-    #pragma omp parallel
+    #pragma omp parallel reduction(+:nev)
     {
         double* blk = (double*)malloc(sizeof(double) * 1296);
         #pragma omp for schedule(dynamic, 4)
@@ -356,6 +358,7 @@ int orc_coulomb_class(int la, int lb, int lc, int ld,
                 prim_of(prim_info[lc], prim_count[lc], K, &ak, C);
                 prim_of(prim_info[ld], prim_count[ld], Lp, &al, D);
                 orc_eri_block(la, lb, lc, ld, ai, A, aj, B, ak, C, al, D, 0.0, blk);
+                ++nev;
                 for (int tab = 0; tab < nab; ++tab) {
                     const double Qij = ab_mat_Q[e0 + tab];
                     const double Sij = ab_pair_data[e0 + tab];
@@ -377,6 +380,7 @@ int orc_coulomb_class(int la, int lb, int lc, int ld,
         free(blk);
     }
     free(beta);
+    if (n_eval) *n_eval += nev;
     return 0;
 }
 
@@ -384,7 +388,9 @@ int orc_coulomb_class(int la, int lb, int lc, int ld,
  * K for one class (la lb | lc ld) over the reference-layout CSR lists.  The
  * ik list holds complete (I,K) blocks (na*nc entries, a-major); every CSR
  * segment of a combined index i = I*na + ca lists the partner primitive blocks
- * of I (nb entries each) in the same order for every ca.
+ * of I (nb entries each) in the same order for every ca.  max_j > 0 limits
+ * each block to its first max_j partner blocks of I (CPU-timing samples only);
+ * *n_eval (if not NULL) is increased by the number of quartets evaluated.
  */
 int orc_exchange_class(int la, int lb, int lc, int ld,
                        const uint32_t* ik_i, const uint32_t* ik_k, uint32_t ik_count,
@@ -398,15 +404,16 @@ int orc_exchange_class(int la, int lb, int lc, int ld,
                        const double* pd_ab, const double* pd_cd,
                        double omega, double thr, int mode,
                        const uint32_t* ik_blocks, uint32_t n_ik_blocks,
-                       double* mat_K)
+                       double* mat_K, uint32_t max_j, unsigned long long* n_eval)
 {
+    unsigned long long nev = 0;
     const int na = ncart_of(la), nb = ncart_of(lb), nc = ncart_of(lc), nd = ncart_of(ld);
     const int nac = na * nc;
     if (ik_count % nac) return -1;
     const uint32_t nblk = ik_count / nac;
     const uint32_t nsel = ik_blocks ? n_ik_blocks : nblk;
 
-    #pragma omp parallel
+    #pragma omp parallel reduction(+:nev)
     {
         double* blk = (double*)malloc(sizeof(double) * 1296);
         #pragma omp for schedule(dynamic, 2)
@@ -420,7 +427,9 @@ int orc_exchange_class(int la, int lb, int lc, int ld,
             prim_of(prim_info[lc], prim_count[lc], K, &ak, C);
             /* segments of I's components and K's components */
             const uint32_t cnt_i = counts_ab[I * na], cnt_k = counts_cd[K * nc];
-            const uint32_t nJ = cnt_i / nb, nL = cnt_k / nd;
+            uint32_t nJ = cnt_i / nb;
+            const uint32_t nL = cnt_k / nd;
+            if (max_j && nJ > max_j) nJ = max_j;
             double acc[36];
             for (int t = 0; t < nac; ++t) acc[t] = 0.0;
             for (uint32_t jb = 0; jb < nJ; ++jb) {
@@ -449,6 +458,7 @@ int orc_exchange_class(int la, int lb, int lc, int ld,
                     double al, Dc[3];
                     prim_of(prim_info[ld], prim_count[ld], Lp, &al, Dc);
                     orc_eri_block(la, lb, lc, ld, ai, A, aj, B, ak, C, al, Dc, omega, blk);
+                    ++nev;
                     for (int ca = 0; ca < na; ++ca)
                         for (int cc = 0; cc < nc; ++cc) {
                             double s_acc = 0.0;
@@ -474,6 +484,7 @@ int orc_exchange_class(int la, int lb, int lc, int ld,
         }
         free(blk);
     }
+    if (n_eval) *n_eval += nev;
     return 0;
 }
 

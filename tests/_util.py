@@ -8,6 +8,7 @@ from __future__ import annotations
 
 import os
 import sys
+import time
 
 import numpy as np
 
@@ -31,8 +32,9 @@ def cart_density(mol, D_sph):
     return T.T @ D_sph @ T
 
 
-def oracle_matJ(fl, D_cart, mode="component", bra_blocks=None, classes=None):
-    """Per bra class mat_J (bra list order) summed over ket classes."""
+def oracle_matJ(fl, D_cart, mode="component", bra_blocks=None, classes=None, n_eval=None, timer=None):
+    """Per bra class mat_J (bra list order) summed over ket classes.  ``timer``
+    (a list) receives the seconds spent inside the oracle kernels only."""
     out = {}
     for ab, cd in (classes or P.j_classes()):
         if ab not in fl.jlists or cd not in fl.jlists:
@@ -45,14 +47,18 @@ def oracle_matJ(fl, D_cart, mode="component", bra_blocks=None, classes=None):
         counts = [t.count if t is not None else 0 for t in fl.prims]
         if ab not in out:
             out[ab] = np.zeros(bra.count)
-        oracle.coulomb_class(la, lb, lc, ld, infos, counts, mD[order], bra.mat_Q, ket.mat_Q[order],
-                             bra.first, bra.second, bra.pair_data,
-                             ket.first[order], ket.second[order], ket.pair_data[order],
-                             fl.thr, mode=mode, bra_blocks=bra_blocks, mat_J=out[ab])
+        args = (mD[order], bra.mat_Q, ket.mat_Q[order], bra.first, bra.second, bra.pair_data,
+                ket.first[order], ket.second[order], ket.pair_data[order])
+        t0 = time.perf_counter()
+        oracle.coulomb_class(la, lb, lc, ld, infos, counts, *args,
+                             fl.thr, mode=mode, bra_blocks=bra_blocks, mat_J=out[ab], n_eval=n_eval)
+        if timer is not None:
+            timer.append(time.perf_counter() - t0)
     return out
 
 
-def oracle_matK(fl, D_cart, mode="component", ik_blocks=None, classes=None, omega=0.0):
+def oracle_matK(fl, D_cart, mode="component", ik_blocks=None, classes=None, omega=0.0, max_j=0, n_eval=None,
+                timer=None):
     out = {}
     infos = [t.info if t is not None else None for t in fl.prims]
     aos = [t.aoinds if t is not None else None for t in fl.prims]
@@ -68,10 +74,13 @@ def oracle_matK(fl, D_cart, mode="component", ik_blocks=None, classes=None, omeg
         if ac not in out:
             out[ac] = np.zeros(ik.i.size)
         A, Cc = fl.kcsr[ab], fl.kcsr[cd]
+        t0 = time.perf_counter()
         oracle.exchange_class(la, lb, lc, ld, ik.i, ik.k, infos, aos, counts, mx, D_cart, fl.naos,
                               A.Q, Cc.Q, A.D_inds, Cc.D_inds, A.displs, Cc.displs, A.counts, Cc.counts,
                               A.pair_data, Cc.pair_data, omega, fl.thr, mode=mode,
-                              ik_blocks=ik_blocks, mat_K=out[ac])
+                              ik_blocks=ik_blocks, mat_K=out[ac], max_j=max_j, n_eval=n_eval)
+        if timer is not None:
+            timer.append(time.perf_counter() - t0)
     return out
 
 

@@ -93,3 +93,45 @@ extern "C" int vlxmi_boys_eval(const double* T, uint32_t n, uint32_t L, const do
     }
     return (int)hipGetLastError();
 }
+
+// Number of primitive shell quartets of one K class that pass the block screen
+// QKmax_IJ * QKmax_KL * maxD_bd > thr (the N_q of the algorithmic FLOP model).
+// qa/qc: partner-block maxima in CSR order (non-increasing within a primitive),
+// *_displ/*_cnt: first block and block count per primitive.  out is added to.
+__global__ void count_k_kernel(const uint32_t* __restrict__ ikI, const uint32_t* __restrict__ ikK, uint32_t nblk,
+                               const double* __restrict__ qa, const uint32_t* __restrict__ qa_displ,
+                               const uint32_t* __restrict__ qa_cnt, const double* __restrict__ qc,
+                               const uint32_t* __restrict__ qc_displ, const uint32_t* __restrict__ qc_cnt,
+                               double lim, unsigned long long* __restrict__ out)
+{
+    const uint32_t b = blockIdx.x;
+    if (b >= nblk) return;
+    const uint32_t I = ikI[b], K = ikK[b];
+    const double* qaI = qa + qa_displ[I];
+    const uint32_t nJ = qa_cnt[I];
+    const double* qcK = qc + qc_displ[K];
+    const uint32_t nL = qc_cnt[K];
+    unsigned long long c = 0;
+    for (uint32_t j = threadIdx.x; j < nJ; j += blockDim.x) {
+        const double q = qaI[j];
+        uint32_t lo = 0, hi = nL;
+        while (lo < hi) {
+            const uint32_t mid = (lo + hi) >> 1;
+            if (q * qcK[mid] > lim) lo = mid + 1; else hi = mid;
+        }
+        c += lo;
+    }
+    if (c) atomicAdd(out, c);
+}
+
+extern "C" int vlxmi_count_k_quartets(const uint32_t* ikI, const uint32_t* ikK, uint32_t nblk,
+                                      const double* qa, const uint32_t* qa_displ, const uint32_t* qa_cnt,
+                                      const double* qc, const uint32_t* qc_displ, const uint32_t* qc_cnt,
+                                      double thr_over_maxD, unsigned long long* out, void* stream)
+{
+    if (nblk == 0) return VLXMI_OK;
+    if (!ikI || !ikK || !qa || !qa_displ || !qa_cnt || !qc || !qc_displ || !qc_cnt || !out) return VLXMI_ERR_ARG;
+    hipLaunchKernelGGL(count_k_kernel, dim3(nblk), dim3(64), 0, (hipStream_t)stream, ikI, ikK, nblk, qa, qa_displ,
+                       qa_cnt, qc, qc_displ, qc_cnt, thr_over_maxD, out);
+    return (int)hipGetLastError();
+}

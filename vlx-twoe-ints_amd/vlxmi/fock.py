@@ -40,6 +40,7 @@ class FockEngine:
         self.thr = float(thr)
         self.omega = float(omega)
         self.target_waves = target_waves
+        self.timer = None          # list of (kernel name, start event, end event) when set
         t0 = time.time()
         self.fl = P.build_lists(mol, self.dev.schwarz, thr=thr, d_bound=d_bound)
         self.setup_lists_s = time.time() - t0
@@ -61,10 +62,14 @@ class FockEngine:
         self.kc = {}
         for pc, cs in fl.kcsr.items():
             self.kc[pc] = dict(displs=d.tensor(cs.displs), counts=d.tensor(cs.counts), Q=d.tensor(cs.Q),
-                               D_inds=d.tensor(cs.D_inds), pair_data=d.tensor(cs.pair_data))
+                               D_inds=d.tensor(cs.D_inds), pair_data=d.tensor(cs.pair_data),
+                               blk_max=d.tensor(cs.blk_max), blk_displs=d.tensor(cs.blk_displs),
+                               blk_counts=d.tensor(cs.blk_counts))
         self.ik = {}
         for pc, ik in fl.iklists.items():
-            self.ik[pc] = dict(i=d.tensor(ik.i), k=d.tensor(ik.k), count=int(ik.i.size))
+            self.ik[pc] = dict(i=d.tensor(ik.i), k=d.tensor(ik.k), count=int(ik.i.size),
+                               I=d.tensor(ik.I.astype(np.uint32)), K=d.tensor(ik.K.astype(np.uint32)),
+                               nblk=int(ik.I.size))
         self._build_assembly()
         # AO index sets per angular type (for maxD_bd)
         self.ao_of_type = []
@@ -141,6 +146,20 @@ class FockEngine:
                             mat_D=mD[ent].contiguous(), count=j["count"])
         return kets
 
+    def _ev_start(self):
+        if self.timer is None:
+            return None
+        e = self.torch.cuda.Event(enable_timing=True)
+        e.record(self.torch.cuda.current_stream(self.dev.device))
+        return e
+
+    def _ev_end(self, e0, name):
+        if e0 is None:
+            return
+        e1 = self.torch.cuda.Event(enable_timing=True)
+        e1.record(self.torch.cuda.current_stream(self.dev.device))
+        self.timer.append((name, e0, e1))
+
     def _ket_split(self, nbra_blocks, nket_blocks):
         tiles = max(1, (nbra_blocks + 63) // 64)
         s = max(1, -(-self.target_waves // tiles))
@@ -170,7 +189,9 @@ class FockEngine:
                      self.dev.boys_table.data_ptr(), self.dev.boys_ft.data_ptr(), self.thr]
             split = self._ket_split(bra["nblk"], self.jl[cd]["nblk"])
             fn = getattr(L, "vlxmi_launch_computeCoulombFock" + cls.upper())
+            ev = self._ev_start()
             check(fn(*args, split, self.dev.stream_ptr()), "computeCoulombFock" + cls.upper())
+            self._ev_end(ev, "computeCoulombFock" + cls.upper())
         return out
 
     def max_D(self, Dc):
@@ -209,8 +230,24 @@ class FockEngine:
                     args.append(Cc[nm].data_ptr())
             args += [self.dev.boys_table.data_ptr(), self.dev.boys_ft.data_ptr(), self.omega, self.thr]
             fn = getattr(L, "vlxmi_launch_computeExchangeFock" + cls.upper())
+            ev = self._ev_start()
             check(fn(*args, self.dev.stream_ptr()), "computeExchangeFock" + cls.upper())
+            self._ev_end(ev, "computeExchangeFock" + cls.upper())
         return out
+
+    def count_k_quartets(self, cls, maxD):
+        """N_q of one K class (primitive shell quartets passing the block screen)."""
+        t = self.torch
+        a, b, c, d = cls
+        ik, A, Cc = self.ik.get(a + c), self.kc.get(a + b), self.kc.get(c + d)
+        if ik is None or A is None or Cc is None or not maxD > 0:
+            return 0
+        out = t.zeros(1, dtype=t.int64, device=self.dev.device)
+        check(lib().vlxmi_count_k_quartets(ik["I"].data_ptr(), ik["K"].data_ptr(), ik["nblk"],
+                                           A["blk_max"].data_ptr(), A["blk_displs"].data_ptr(), A["blk_counts"].data_ptr(),
+                                           Cc["blk_max"].data_ptr(), Cc["blk_displs"].data_ptr(), Cc["blk_counts"].data_ptr(),
+                                           self.thr / maxD, out.data_ptr(), self.dev.stream_ptr()), "count_k_quartets")
+        return int(out.item())
 
     def build(self, D_sph, want_k=True):
         """J and K in the output basis for density D (device tensors)."""

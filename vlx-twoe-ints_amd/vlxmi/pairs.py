@@ -195,6 +195,9 @@ class KCSR:
     D_inds: np.ndarray       # uint32 (nnz,)
     pair_data: np.ndarray    # float64 (nnz,)
     qmax_prim: np.ndarray    # (count_a,) max QKmax over partners of I
+    blk_max: np.ndarray = None      # (nblocks,) QKmax_IJ of every partner block, CSR order
+    blk_displs: np.ndarray = None   # (count_a,) first partner block of I in blk_max
+    blk_counts: np.ndarray = None   # (count_a,) number of partner blocks of I
 
 
 @dataclass
@@ -351,7 +354,9 @@ def _make_kcsr(fl: FockLists):
                 np.maximum.at(qmax_prim, I, qm)
             fl.kcsr["spd"[la] + "spd"[lb]] = KCSR(
                 la, lb, np.ascontiguousarray(displs, dtype=np.uint32),
-                np.ascontiguousarray(counts, dtype=np.uint32), Qe, De, Se, qmax_prim)
+                np.ascontiguousarray(counts, dtype=np.uint32), Qe, De, Se, qmax_prim,
+                np.ascontiguousarray(qm), np.ascontiguousarray(start, dtype=np.uint32),
+                np.ascontiguousarray(nJ, dtype=np.uint32))
 
 
 def _make_iklists(fl: FockLists):

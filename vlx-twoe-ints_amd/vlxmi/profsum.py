@@ -1,0 +1,62 @@
+"""Summarise a rocprofv3 ``--kernel-trace --stats`` result as a Markdown table.
+
+Reads either the SQLite database rocprofv3 writes by default
+(``*_results.db``, view ``top_kernels``) or a ``*kernel_stats.csv`` file.
+
+    python -m vlxmi.profsum RESULT [--command "the profiled command"] [--top N]
+"""
+
+from __future__ import annotations
+
+import argparse
+import csv
+import sqlite3
+
+
+def _short(name: str) -> str:
+    return name.split("(")[0]
+
+
+def rows_from(path: str):
+    if path.endswith(".db"):
+        con = sqlite3.connect(path)
+        cur = con.cursor()
+        out = [(_short(n), int(c), float(t), float(a), float(p))
+               for n, c, t, a, p in cur.execute("select name,total_calls,total_duration,average,percentage from top_kernels")]
+        con.close()
+        return out
+    out = []
+    with open(path) as f:
+        for r in csv.DictReader(f):
+            out.append((_short(r["Name"]), int(r["Calls"]), float(r["TotalDurationNs"]), float(r["AverageNs"]),
+                        float(r["Percentage"])))
+    return out
+
+
+def markdown(path: str, command: str = "", top: int = 30) -> str:
+    rows = sorted(rows_from(path), key=lambda r: -r[2])
+    total = sum(r[2] for r in rows)
+    o = []
+    if command:
+        o.append("Command: `%s`" % command)
+        o.append("")
+    o.append("| kernel | calls | total ms | avg ms | % |")
+    o.append("|---|---:|---:|---:|---:|")
+    for n, c, t, a, p in rows[:top]:
+        o.append("| %s | %d | %.3f | %.3f | %.2f |" % (n, c, t * 1e-6, a * 1e-6, p))
+    o.append("")
+    o.append("All kernels: %d, total %.1f ms." % (len(rows), total * 1e-6))
+    return "\n".join(o) + "\n"
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("result")
+    ap.add_argument("--command", default="")
+    ap.add_argument("--top", type=int, default=30)
+    a = ap.parse_args()
+    print(markdown(a.result, a.command, a.top), end="")
+
+
+if __name__ == "__main__":
+    main()

@@ -60,6 +60,7 @@ def lib():
     L.vlxmi_status_string.restype = C.c_char_p
     L.vlxmi_status_string.argtypes = [C.c_int]
     L.vlxmi_version.restype = C.c_int
+    L.vlxmi_count_k_quartets.argtypes = [C.c_void_p, C.c_void_p, C.c_uint32] + [C.c_void_p] * 6 + [C.c_double, C.c_void_p, C.c_void_p]
     _lib = L
     return L
 
@@ -73,7 +74,7 @@ def check(rc: int, what: str = ""):
 def exported_symbols():
     """Names every C-ABI entry point the header declares."""
     names = ["vlxmi_version", "vlxmi_boys_table_size", "vlxmi_boys_table_fill", "vlxmi_status_string",
-             "vlxmi_boys_eval"]
+             "vlxmi_boys_eval", "vlxmi_count_k_quartets"]
     names += ["vlxmi_launch_computeCoulombFock" + (ab + cd).upper() for ab, cd in j_classes()]
     names += ["vlxmi_launch_computeExchangeFock" + c.upper() for c in k_classes()]
     names += ["vlxmi_schwarz_" + pc for pc in PAIR_CLASSES]

@@ -21,7 +21,8 @@ def rows_from(path: str):
     if path.endswith(".db"):
         con = sqlite3.connect(path)
         cur = con.cursor()
-        out = [(_short(n), int(c), float(t), float(a), float(p))
+        # the top_kernels view reports microseconds; rows are kept in nanoseconds
+        out = [(_short(n), int(c), 1e3 * float(t), 1e3 * float(a), float(p))
                for n, c, t, a, p in cur.execute("select name,total_calls,total_duration,average,percentage from top_kernels")]
         con.close()
         return out

@@ -107,22 +107,29 @@ def main():
     torch.cuda.synchronize()
     setup_s = time.time() - t0
     Dd = eng.dev.tensor(D)
-    for _ in range(a.warmup):
-        J, K = eng.build(Dd)
-    torch.cuda.synchronize()
 
-    eng.timer = []
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    for _ in range(a.steps):
-        J, K = eng.build(Dd)
-    torch.cuda.synchronize()
-    dt = time.perf_counter() - t0
-    timer, eng.timer = eng.timer, None
-    per = {}
-    for name, e0, e1 in timer:
-        per.setdefault(name, []).append(e0.elapsed_time(e1) * 1e-3)
+    def timed(precision):
+        eng.precision = precision
+        for _ in range(a.warmup):
+            eng.build(Dd)
+        torch.cuda.synchronize()
+        eng.timer = []
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(a.steps):
+            J, K = eng.build(Dd)
+        torch.cuda.synchronize()
+        dt = time.perf_counter() - t0
+        timer, eng.timer = eng.timer, None
+        per = {}
+        for name, e0, e1 in timer:
+            per.setdefault(name, []).append(e0.elapsed_time(e1) * 1e-3)
+        return J, K, dt, per
+
+    J64, K64, dt64, per64 = timed("fp64")
+    J, K, dt, per = timed("mixed")
     kern_total = sum(sum(v) for v in per.values()) / a.steps
+    EJ64, EK64 = eng.energies(Dd, J64, K64)
     EJ, EK = eng.energies(Dd, J, K)
 
     # algorithmic FLOPs of every class launch
@@ -137,15 +144,22 @@ def main():
     for cls in P.k_classes():
         nq = eng.count_k_quartets(cls, maxD.get(cls[1] + cls[3], 0.0))
         W["computeExchangeFock" + cls.upper()] = (nq, nq * model["K"][cls]["total"])
-    total_flop = sum(v[1] for v in W.values())
-    dom = max(per, key=lambda k: sum(per[k]))
-    dom_t = sum(per[dom]) / len(per[dom])
+    for k in list(W):
+        W[k + "_mixed"] = W[k]
+    total_flop = sum(v[1] for k, v in W.items() if not k.endswith("_mixed"))
+    dom = max(per64, key=lambda k: sum(per64[k]))
+    dom_t = sum(per64[dom]) / len(per64[dom])
     achieved = W[dom][1] / dom_t
+    dom_m = max(per, key=lambda k: sum(per[k]))
+    dom_mt = sum(per[dom_m]) / len(per[dom_m])
     top = sorted(per, key=lambda k: -sum(per[k]))[:6]
+    dJ = float((J - J64).abs().max().item())
+    dK = float((K - K64).abs().max().item())
 
     if a.per_class_out:
         rows = {k: {"avg_s": sum(v) / len(v), "quartets": W.get(k, (0, 0))[0], "flop": W.get(k, (0, 0))[1],
-                    "tflops": W.get(k, (0, 0))[1] / (sum(v) / len(v)) / 1e12} for k, v in per.items()}
+                    "tflops": W.get(k, (0, 0))[1] / (sum(v) / len(v)) / 1e12}
+                for pp in (per64, per) for k, v in pp.items()}
         with open(a.per_class_out, "w") as f:
             json.dump(rows, f, indent=1, sort_keys=True)
 
@@ -155,7 +169,7 @@ def main():
 
     ms = 1e3 * dt / a.steps
     out = {
-        "metric": "J+K Fock builds/s (plain FP64; mixed precision not built yet)",
+        "metric": "J+K Fock builds/s, mixed precision (FP32 VRR/HRR below Schwarz x |D| 1e-6) vs plain FP64",
         "value": a.steps / dt,
         "unit": "builds/s",
         "n_gpus": 1,
@@ -165,22 +179,26 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64",
+        "dtype": "f64+f32",
         "data": "synthetic",
         "config": {"workload": "lattice64: 64 atoms, 4x4x4 lattice 2.5 bohr +-0.3 jitter, per atom 3s(3)+2p(2)+1 sph d(1), "
                                "nbf 896 (960 Cartesian), D=CC^T C(896x224)~N(0,0.01), thr 1e-12, classes ss..dd (36 J + 54 K)",
-                   "precision": "fp64", "nbf": mol.nbf, "naos_cart": mol.naos_cart},
-        "roofline": {"bound": "valu", "kernel": dom, "achieved": achieved / 1e12, "peak": PEAK_FP64_VALU / 1e12,
-                     "unit": "TFLOP/s", "frac": achieved / PEAK_FP64_VALU, "traffic": None,
-                     "algorithmic_flop_per_launch": W[dom][1], "quartets_per_launch": W[dom][0],
-                     "avg_launch_s": dom_t},
+                   "precision": "mixed (fp32 threshold 1e-6)", "nbf": mol.nbf, "naos_cart": mol.naos_cart},
+        "roofline": {"bound": "valu", "kernel": dom_m, "achieved": W[dom_m][1] / dom_mt / 1e12,
+                     "peak": PEAK_FP64_VALU / 1e12, "unit": "TFLOP/s", "frac": W[dom_m][1] / dom_mt / PEAK_FP64_VALU,
+                     "traffic": None, "algorithmic_flop_per_launch": W[dom_m][1],
+                     "quartets_per_launch": W[dom_m][0], "avg_launch_s": dom_mt,
+                     "fp64_path": {"kernel": dom, "achieved": achieved / 1e12, "frac": achieved / PEAK_FP64_VALU,
+                                   "avg_launch_s": dom_t}},
         "cpu_baseline": cpu,
-        "fp64_builds_per_s": a.steps / dt,
-        "mixed_builds_per_s": None,
+        "fp64_builds_per_s": a.steps / dt64,
+        "mixed_builds_per_s": a.steps / dt,
+        "mixed_speedup": dt64 / dt,
+        "mixed_error_vs_fp64": {"max_abs_dJ": dJ, "max_abs_dK": dK, "dE_J": EJ - EJ64, "dE_K": EK - EK64},
         "whole_build": {"algorithmic_flop": total_flop, "kernel_s": kern_total,
                         "tflops_over_kernels": total_flop / kern_total / 1e12,
                         "tflops_over_wall": total_flop / (dt / a.steps) / 1e12},
-        "E_J": EJ, "E_K": EK, "setup_s": setup_s,
+        "E_J": EJ64, "E_K": EK64, "setup_s": setup_s,
         "top_kernels_ms": {k: 1e3 * sum(per[k]) / len(per[k]) for k in top},
     }
     print(json.dumps(out), flush=True)

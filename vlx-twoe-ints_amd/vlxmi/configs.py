@@ -152,5 +152,15 @@ def serve_iteration_density(D0, m, it):
     return D0 + 0.5 * (E + E.T)
 
 
+def make_lattice4():
+    """2x2x1 lattice with the lattice basis (4 atoms, nbf 56): small enough for
+    the full CPU oracle, and it contains every class through (dd|dd)."""
+    rng = np.random.default_rng(990)
+    mol = lattice_molecule(rng, (2, 2, 1), name="lattice4")
+    D = density(rng, mol.nbf, mol.nbf // 4, 0.1)
+    return mol, D
+
+
 def make_workload(name):
-    return {"small8": make_small8, "lattice64": make_lattice64, "lattice256": make_lattice256}[name]()
+    return {"small8": make_small8, "lattice4": make_lattice4, "lattice64": make_lattice64,
+            "lattice256": make_lattice256}[name]()

@@ -32,7 +32,8 @@ from .runtime import Device, check, lib
 
 
 class FockEngine:
-    def __init__(self, mol, thr=1e-12, dev=None, d_bound=100.0, omega=0.0, target_waves=2048):
+    def __init__(self, mol, thr=1e-12, dev=None, d_bound=100.0, omega=0.0, target_waves=2048,
+                 precision="fp64", fp32_threshold=1e-6):
         self.dev = dev or Device()
         t = self.dev.torch
         self.torch = t
@@ -41,6 +42,11 @@ class FockEngine:
         self.omega = float(omega)
         self.target_waves = target_waves
         self.timer = None          # list of (kernel name, start event, end event) when set
+        # "fp64": plain path; "mixed": quartets whose Schwarz x |D| bound is below
+        # fp32_threshold run the VRR/HRR in FP32 (Boys, prefactors, sums in FP64)
+        assert precision in ("fp64", "mixed")
+        self.precision = precision
+        self.fp32_threshold = float(fp32_threshold)
         t0 = time.time()
         self.fl = P.build_lists(mol, self.dev.schwarz, thr=thr, d_bound=d_bound)
         self.setup_lists_s = time.time() - t0
@@ -188,10 +194,13 @@ class FockEngine:
                      ket["first"].data_ptr(), ket["second"].data_ptr(), ket["pair_data"].data_ptr(), ket["count"],
                      self.dev.boys_table.data_ptr(), self.dev.boys_ft.data_ptr(), self.thr]
             split = self._ket_split(bra["nblk"], self.jl[cd]["nblk"])
-            fn = getattr(L, "vlxmi_launch_computeCoulombFock" + cls.upper())
+            name = "computeCoulombFock" + cls.upper() + ("_mixed" if self.precision == "mixed" else "")
+            if self.precision == "mixed":
+                args.append(self.fp32_threshold)
+            fn = getattr(L, "vlxmi_launch_" + name)
             ev = self._ev_start()
-            check(fn(*args, split, self.dev.stream_ptr()), "computeCoulombFock" + cls.upper())
-            self._ev_end(ev, "computeCoulombFock" + cls.upper())
+            check(fn(*args, split, self.dev.stream_ptr()), name)
+            self._ev_end(ev, name)
         return out
 
     def max_D(self, Dc):
@@ -229,10 +238,13 @@ class FockEngine:
                 if cd != ab:
                     args.append(Cc[nm].data_ptr())
             args += [self.dev.boys_table.data_ptr(), self.dev.boys_ft.data_ptr(), self.omega, self.thr]
-            fn = getattr(L, "vlxmi_launch_computeExchangeFock" + cls.upper())
+            name = "computeExchangeFock" + cls.upper() + ("_mixed" if self.precision == "mixed" else "")
+            if self.precision == "mixed":
+                args.append(self.fp32_threshold)
+            fn = getattr(L, "vlxmi_launch_" + name)
             ev = self._ev_start()
-            check(fn(*args, self.dev.stream_ptr()), "computeExchangeFock" + cls.upper())
-            self._ev_end(ev, "computeExchangeFock" + cls.upper())
+            check(fn(*args, self.dev.stream_ptr()), name)
+            self._ev_end(ev, name)
         return out
 
     def count_k_quartets(self, cls, maxD):

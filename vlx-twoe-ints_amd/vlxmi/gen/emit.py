@@ -153,7 +153,7 @@ def _strip_restrict(sig):
 # --------------------------------------------------------------------------
 
 def emit_j(cls, mixed=False):
-    """Source of computeCoulombFock{CLS} and its C-ABI launcher."""
+    """Source of computeCoulombFock{CLS} (or its _mixed variant) and its C-ABI launcher."""
     a, b, c, d = cls
     la, lb, lc, ld = LV[a], LV[b], LV[c], LV[d]
     na, nb, nc, nd = NC[a], NC[b], NC[c], NC[d]
@@ -162,8 +162,8 @@ def emit_j(cls, mixed=False):
     L = la + lb + lc + ld
     rec = 12 + 2 * ncd
     rec += rec % 2
-    name = "computeCoulombFock" + cls.upper()
-    sig = j_signature(cls)
+    name = "computeCoulombFock" + cls.upper() + ("_mixed" if mixed else "")
+    sig = j_signature(cls) + (["const double fp32_threshold"] if mixed else [])
     ty = "double"
     o = []
     o.append("__global__ void __launch_bounds__(TILE_SIZE_J)")
@@ -253,22 +253,33 @@ def emit_j(cls, mixed=False):
     o.append("                const double Q_x = R[3], Q_y = R[4], Q_z = R[5];")
     o.append("                const double QC_x = R[6], QC_y = R[7], QC_z = R[8];")
     o.append("                const double CD_x = R[9], CD_y = R[10], CD_z = R[11];")
-    body = quartet_prologue(L, ty_geo=ty, ind="                ")
-    o += body
-    dag = QuartetDAG(la, lb, lc, ld, ty=ty, rule_vrr="first", rule_hrr="first")
-    dig = []
-    for ic, cc in enumerate(tri(lc)):
-        for idd, dd in enumerate(tri(ld)):
-            tcd = ic * nd + idd
-            dig.append("const double Dp%d = R[%d], Qd%d = R[%d];" % (tcd, 12 + tcd, tcd, 12 + ncd + tcd))
-            for ia, aa in enumerate(tri(la)):
-                for ib, bbb in enumerate(tri(lb)):
-                    tab = ia * nb + ib
-                    x = dag.X(aa, bbb, cc, dd)
-                    dag.lines.append("acc[%d] += (Qd%d > tau[%d]) ? (%s * Dp%d) : 0.0;" % (tab, tcd, tab, x, tcd))
-    # the digestion loads must precede their uses: emit loads first
-    o += ["                " + x for x in dig]
-    o += ["                " + x for x in _dag_lines(dag)]
+    def jbody(tyq, ind):
+        lines = quartet_prologue(L, ty_geo=tyq, ind=ind)
+        dag = QuartetDAG(la, lb, lc, ld, ty=tyq, rule_vrr="first", rule_hrr="first")
+        dig = []
+        for ic, cc in enumerate(tri(lc)):
+            for idd, dd in enumerate(tri(ld)):
+                tcd = ic * nd + idd
+                dig.append("const double Dp%d = R[%d], Qd%d = R[%d];" % (tcd, 12 + tcd, tcd, 12 + ncd + tcd))
+                for ia, aa in enumerate(tri(la)):
+                    for ib, bbb in enumerate(tri(lb)):
+                        tab = ia * nb + ib
+                        x = dag.X(aa, bbb, cc, dd)
+                        xv = x if tyq == "double" else "double(%s)" % x
+                        dag.lines.append("acc[%d] += (Qd%d > tau[%d]) ? (%s * Dp%d) : 0.0;" % (tab, tcd, tab, xv, tcd))
+        lines += [ind + x for x in dig]
+        lines += [ind + x for x in _dag_lines(dag)]
+        return lines
+
+    if mixed:
+        o.append("                const bool lo = qmax * R[0] < fp32_threshold;")
+        o.append("                if (__all(lo)) {")
+        o += jbody("float", "                    ")
+        o.append("                } else {")
+        o += jbody("double", "                    ")
+        o.append("                }")
+    else:
+        o += jbody("double", "                ")
     o.append("            }")
     o.append("        }")
     o.append("    }")
@@ -304,7 +315,7 @@ def emit_j(cls, mixed=False):
 # K kernel
 # --------------------------------------------------------------------------
 
-def emit_k(cls):
+def emit_k(cls, mixed=False):
     a, b, c, d = cls
     la, lb, lc, ld = LV[a], LV[b], LV[c], LV[d]
     na, nb, nc, nd = NC[a], NC[b], NC[c], NC[d]
@@ -313,8 +324,8 @@ def emit_k(cls):
     L = la + lb + lc + ld
     rec = 12 + 2 * ncd
     rec += rec % 2
-    name = "computeExchangeFock" + cls.upper()
-    sig = k_signature(cls)
+    name = "computeExchangeFock" + cls.upper() + ("_mixed" if mixed else "")
+    sig = k_signature(cls) + (["const double fp32_threshold"] if mixed else [])
     ty = "double"
     o = []
     o.append("__global__ void __launch_bounds__(TILE_SIZE_K)")
@@ -439,30 +450,43 @@ def emit_k(cls):
     o.append("                    const double Q_x = R[3], Q_y = R[4], Q_z = R[5];")
     o.append("                    const double QC_x = R[6], QC_y = R[7], QC_z = R[8];")
     o.append("                    const double CD_x = R[9], CD_y = R[10], CD_z = R[11];")
-    o += quartet_prologue(L, ty_geo=ty, omega=True, ind="                    ")
-    pre = []
-    for ibb in range(nb):
-        for idd in range(nd):
-            pre.append("const double Dj%d_%d = mat_D_full_AO[(size_t)cgJ[%d] * naos + ket_cg[s * ND + %d]];" % (ibb, idd, ibb, idd))
-    for icc in range(nc):
-        for idd in range(nd):
-            tcd = icc * nd + idd
-            pre.append("const double Sc%d = R[%d], Qk%d = R[%d];" % (tcd, 12 + tcd, tcd, 12 + ncd + tcd))
-    dag = QuartetDAG(la, lb, lc, ld, ty=ty, rule_vrr="first", rule_hrr="first")
-    # digestion: acc[a,c] += Sab * sum_{b,d} mask * X * Sc * D
-    for ic, cc in enumerate(tri(lc)):
-        for ia, aa in enumerate(tri(la)):
-            tac = ia * nc + ic
-            for ib, bbx in enumerate(tri(lb)):
-                tab = ia * nb + ib
-                terms = []
-                for idd, dd in enumerate(tri(ld)):
-                    tcd = ic * nd + idd
-                    x = dag.X(aa, bbx, cc, dd)
-                    terms.append("((Qk%d > tau[%d]) ? (%s * Sc%d * Dj%d_%d) : 0.0)" % (tcd, tab, x, tcd, ib, idd))
-                dag.lines.append("acc[%d] += Sab[%d] * (%s);" % (tac, tab, " + ".join(terms)))
-    o += ["                    " + x for x in pre]
-    o += ["                    " + x for x in _dag_lines(dag)]
+    def kbody(tyq, ind):
+        lines = quartet_prologue(L, ty_geo=tyq, omega=True, ind=ind)
+        pre = []
+        for ibb in range(nb):
+            for idd in range(nd):
+                pre.append("const double Dj%d_%d = mat_D_full_AO[(size_t)cgJ[%d] * naos + ket_cg[s * ND + %d]];" % (ibb, idd, ibb, idd))
+        for icc in range(nc):
+            for idd in range(nd):
+                tcd = icc * nd + idd
+                pre.append("const double Sc%d = R[%d], Qk%d = R[%d];" % (tcd, 12 + tcd, tcd, 12 + ncd + tcd))
+        dag = QuartetDAG(la, lb, lc, ld, ty=tyq, rule_vrr="first", rule_hrr="first")
+        # digestion: acc[a,c] += Sab * sum_{b,d} mask * X * Sc * D  (FP64 accumulation)
+        for ic, cc in enumerate(tri(lc)):
+            for ia, aa in enumerate(tri(la)):
+                tac = ia * nc + ic
+                for ib, bbx in enumerate(tri(lb)):
+                    tab = ia * nb + ib
+                    terms = []
+                    for idd, dd in enumerate(tri(ld)):
+                        tcd = ic * nd + idd
+                        x = dag.X(aa, bbx, cc, dd)
+                        xv = x if tyq == "double" else "double(%s)" % x
+                        terms.append("((Qk%d > tau[%d]) ? (%s * Sc%d * Dj%d_%d) : 0.0)" % (tcd, tab, xv, tcd, ib, idd))
+                    dag.lines.append("acc[%d] += Sab[%d] * (%s);" % (tac, tab, " + ".join(terms)))
+        lines += [ind + x for x in pre]
+        lines += [ind + x for x in _dag_lines(dag)]
+        return lines
+
+    if mixed:
+        o.append("                    const bool lo = qij * R[0] * maxD < fp32_threshold;")
+        o.append("                    if (__all(lo)) {")
+        o += kbody("float", "                        ")
+        o.append("                    } else {")
+        o += kbody("double", "                        ")
+        o.append("                    }")
+    else:
+        o += kbody("double", "                    ")
     o.append("                }")
     o.append("            }")
     o.append("        }")

@@ -152,8 +152,12 @@ def _strip_restrict(sig):
 # J kernel
 # --------------------------------------------------------------------------
 
-def emit_j(cls, mixed=False):
-    """Source of computeCoulombFock{CLS} (or its _mixed variant) and its C-ABI launcher."""
+def emit_j(cls, mixed=False, batched=False):
+    """Source of computeCoulombFock{CLS} (or its _mixed / _batched variants) and its
+    C-ABI launcher.  The batched variant is persistent: workgroup w processes
+    work items w, w + gridDim.x, ...; item t = (bra block begin, bra block end
+    (<= begin + 64), ket block begin, ket block end) over lists that concatenate
+    many molecules; each bra tile must appear in at most one item (plain +=)."""
     a, b, c, d = cls
     la, lb, lc, ld = LV[a], LV[b], LV[c], LV[d]
     na, nb, nc, nd = NC[a], NC[b], NC[c], NC[d]
@@ -162,8 +166,9 @@ def emit_j(cls, mixed=False):
     L = la + lb + lc + ld
     rec = 12 + 2 * ncd
     rec += rec % 2
-    name = "computeCoulombFock" + cls.upper() + ("_mixed" if mixed else "")
-    sig = j_signature(cls) + (["const double fp32_threshold"] if mixed else [])
+    name = "computeCoulombFock" + cls.upper() + ("_batched" if batched else "") + ("_mixed" if mixed else "")
+    sig = j_signature(cls) + (["const uint32_t* __restrict__ work", "const uint32_t nwork"] if batched else []) \
+        + (["const double fp32_threshold"] if mixed else [])
     ty = "double"
     o = []
     o.append("__global__ void __launch_bounds__(TILE_SIZE_J)")
@@ -175,12 +180,21 @@ def emit_j(cls, mixed=False):
     o.append("    const uint32_t lane = threadIdx.x;")
     o.append("    const uint32_t nbra = %s_prim_pair_count_local / NAB;" % ab)
     o.append("    const uint32_t nket = %s_prim_pair_count / NCD;" % cd)
-    o.append("    const uint32_t bb = blockIdx.x * TILE_SIZE_J + lane;")
-    o.append("    const uint32_t chunk = (nket + gridDim.y - 1) / gridDim.y;")
-    o.append("    const uint32_t k_begin = blockIdx.y * chunk;")
-    o.append("    const uint32_t k_end = (k_begin + chunk < nket) ? (k_begin + chunk) : nket;")
+    if batched:
+        o.append("    for (uint32_t w = blockIdx.x; w < nwork; w += gridDim.x) {")
+        o.append("    const uint32_t bra_end = work[4 * w + 1] < nbra ? work[4 * w + 1] : nbra;")
+        o.append("    const uint32_t bb = work[4 * w] + lane;")
+        o.append("    const uint32_t k_begin = work[4 * w + 2];")
+        o.append("    const uint32_t k_end = work[4 * w + 3] < nket ? work[4 * w + 3] : nket;")
+        o.append("    const bool own = bb < bra_end;")
+    else:
+        o.append("    const uint32_t bb = blockIdx.x * TILE_SIZE_J + lane;")
+        o.append("    const uint32_t chunk = (nket + gridDim.y - 1) / gridDim.y;")
+        o.append("    const uint32_t k_begin = blockIdx.y * chunk;")
+        o.append("    const uint32_t k_end = (k_begin + chunk < nket) ? (k_begin + chunk) : nket;")
+        o.append("    const bool own = bb < nbra;")
     o.append("    const double thr = eri_threshold;")
-    o.append("    bool active = bb < nbra;")
+    o.append("    bool active = own;")
     o.append("    double zeta = 1.0, inv_S1 = 1.0, P_x = 0.0, P_y = 0.0, P_z = 0.0;")
     o.append("    double PA_x = 0.0, PA_y = 0.0, PA_z = 0.0, AB_x = 0.0, AB_y = 0.0, AB_z = 0.0;")
     o.append("    double qmax = 0.0;")
@@ -283,28 +297,42 @@ def emit_j(cls, mixed=False):
     o.append("            }")
     o.append("        }")
     o.append("    }")
-    o.append("    if (bb < nbra) {")
+    o.append("    if (own) {")
     o.append("        const uint32_t e0 = bb * NAB;")
     o.append("        #pragma unroll")
     o.append("        for (int t = 0; t < NAB; ++t) {")
     o.append("            const double v = %s_pair_data_local[e0 + t] * acc[t];" % ab)
-    o.append("            if (gridDim.y > 1) atomicAdd(&mat_J[e0 + t], v); else mat_J[e0 + t] += v;")
+    if batched:
+        o.append("            mat_J[e0 + t] += v;")
+    else:
+        o.append("            if (gridDim.y > 1) atomicAdd(&mat_J[e0 + t], v); else mat_J[e0 + t] += v;")
     o.append("        }")
     o.append("    }")
+    if batched:
+        o.append("    }")
     o.append("}")
     o.append("")
     # launcher
     lsig = _strip_restrict(sig)
     args = _argnames(sig)
-    o.append('extern "C" int vlxmi_launch_%s(%s,\n    uint32_t ket_split, void* stream)' % (name, ",\n    ".join(lsig)))
+    if batched:
+        o.append('extern "C" int vlxmi_launch_%s(%s,\n    uint32_t n_workgroups, void* stream)' % (name, ",\n    ".join(lsig)))
+    else:
+        o.append('extern "C" int vlxmi_launch_%s(%s,\n    uint32_t ket_split, void* stream)' % (name, ",\n    ".join(lsig)))
     o.append("{")
     o.append("    if (%s_prim_pair_count_local %% %d != 0 || %s_prim_pair_count %% %d != 0) return VLXMI_ERR_LAYOUT;" % (ab, nab, cd, ncd))
     o.append("    const uint32_t nbra = %s_prim_pair_count_local / %d, nket = %s_prim_pair_count / %d;" % (ab, nab, cd, ncd))
     o.append("    if (nbra == 0 || nket == 0) return VLXMI_OK;")
     o.append("    if (!mat_J || !boys_func_table) return VLXMI_ERR_ARG;")
-    o.append("    if (ket_split == 0) ket_split = 1;")
-    o.append("    if (ket_split > nket) ket_split = nket;")
-    o.append("    dim3 grid((nbra + TILE_SIZE_J - 1) / TILE_SIZE_J, ket_split);")
+    if batched:
+        o.append("    if (nwork == 0) return VLXMI_OK;")
+        o.append("    if (!work) return VLXMI_ERR_ARG;")
+        o.append("    if (n_workgroups == 0 || n_workgroups > nwork) n_workgroups = nwork;")
+        o.append("    dim3 grid(n_workgroups);")
+    else:
+        o.append("    if (ket_split == 0) ket_split = 1;")
+        o.append("    if (ket_split > nket) ket_split = nket;")
+        o.append("    dim3 grid((nbra + TILE_SIZE_J - 1) / TILE_SIZE_J, ket_split);")
     o.append("    hipLaunchKernelGGL(%s, grid, dim3(TILE_SIZE_J), 0, (hipStream_t)stream, %s);" % (name, ", ".join(args)))
     o.append("    return (int)hipGetLastError();")
     o.append("}")
@@ -315,7 +343,13 @@ def emit_j(cls, mixed=False):
 # K kernel
 # --------------------------------------------------------------------------
 
-def emit_k(cls, mixed=False):
+def emit_k(cls, mixed=False, batched=False):
+    """Source of computeExchangeFock{CLS} (or its _mixed / _batched variants) and
+    its launcher.  The batched variant is persistent over a list of (i,k) blocks
+    of lists that concatenate many molecules; per block, blk_mol gives the
+    molecule, whose density lives at mat_D_full_AO + mol_D_off[m] (mol_naos[m]
+    Cartesian AOs, AO indices offset by mol_ao_off[m]) and whose maxD_bd is
+    mol_maxD[m]."""
     a, b, c, d = cls
     la, lb, lc, ld = LV[a], LV[b], LV[c], LV[d]
     na, nb, nc, nd = NC[a], NC[b], NC[c], NC[d]
@@ -324,8 +358,11 @@ def emit_k(cls, mixed=False):
     L = la + lb + lc + ld
     rec = 12 + 2 * ncd
     rec += rec % 2
-    name = "computeExchangeFock" + cls.upper() + ("_mixed" if mixed else "")
-    sig = k_signature(cls) + (["const double fp32_threshold"] if mixed else [])
+    name = "computeExchangeFock" + cls.upper() + ("_batched" if batched else "") + ("_mixed" if mixed else "")
+    bsig = ["const uint32_t* __restrict__ blk_list", "const uint32_t nlist", "const uint32_t* __restrict__ blk_mol",
+            "const double* __restrict__ mol_maxD", "const unsigned long long* __restrict__ mol_D_off",
+            "const uint32_t* __restrict__ mol_naos", "const uint32_t* __restrict__ mol_ao_off"]
+    sig = k_signature(cls) + (bsig if batched else []) + (["const double fp32_threshold"] if mixed else [])
     ty = "double"
     o = []
     o.append("__global__ void __launch_bounds__(TILE_SIZE_K)")
@@ -336,10 +373,23 @@ def emit_k(cls, mixed=False):
     o.append("    __shared__ double ket[KT * REC];")
     o.append("    __shared__ uint32_t ket_cg[KT * ND];")
     o.append("    const uint32_t lane = threadIdx.x;")
-    o.append("    const uint32_t blk = blockIdx.x;")
-    o.append("    if ((blk + 1) * NAC > pair_inds_count_for_K_%s) return;" % ac)
-    o.append("    const double maxD = %s_max_D;" % bd)
-    o.append("    if (!(maxD > 0.0)) return;")
+    if batched:
+        o.append("    (void)%s_max_D;" % bd)
+        o.append("    for (uint32_t w = blockIdx.x; w < nlist; w += gridDim.x) {")
+        o.append("    const uint32_t blk = blk_list[w];")
+        o.append("    if ((blk + 1) * NAC > pair_inds_count_for_K_%s) continue;" % ac)
+        o.append("    const uint32_t mol = blk_mol[blk];")
+        o.append("    const double maxD = mol_maxD[mol];")
+        o.append("    if (!(maxD > 0.0)) continue;")
+        o.append("    const double* __restrict__ Dm = mat_D_full_AO + mol_D_off[mol];")
+        o.append("    const uint32_t nao = mol_naos[mol], aoff = mol_ao_off[mol];")
+    else:
+        o.append("    const uint32_t blk = blockIdx.x;")
+        o.append("    if ((blk + 1) * NAC > pair_inds_count_for_K_%s) return;" % ac)
+        o.append("    const double maxD = %s_max_D;" % bd)
+        o.append("    if (!(maxD > 0.0)) return;")
+        o.append("    const double* __restrict__ Dm = mat_D_full_AO;")
+        o.append("    const uint32_t nao = naos, aoff = 0;")
     o.append("    const double thr = eri_threshold;")
     o.append("    const double thr_eff = thr / maxD;")
     o.append("    const uint32_t e0 = blk * NAC;")
@@ -395,9 +445,9 @@ def emit_k(cls, mixed=False):
     o.append("            for (int cb = 0; cb < NB; ++cb) {")
     o.append("                const uint32_t jj = D_inds_K_%s[displ_i[0] + jp * NB + cb];" % ab)
     if b == "s":
-        o.append("                cgJ[cb] = %s_prim_aoinds[jj];" % b)
+        o.append("                cgJ[cb] = %s_prim_aoinds[jj] - aoff;" % b)
     else:
-        o.append("                cgJ[cb] = %s_prim_aoinds[(jj / NB) + %s_prim_count * (jj %% NB)];" % (b, b))
+        o.append("                cgJ[cb] = %s_prim_aoinds[(jj / NB) + %s_prim_count * (jj %% NB)] - aoff;" % (b, b))
     o.append("            }")
     o.append("        }")
     o.append("        for (uint32_t t0 = 0; t0 < nL; t0 += KT) {")
@@ -433,9 +483,9 @@ def emit_k(cls, mixed=False):
     o.append("                    for (int cdd = 0; cdd < ND; ++cdd) {")
     o.append("                        const uint32_t ll = D_inds_K_%s[displ_k[0] + lp * ND + cdd];" % cd)
     if d == "s":
-        o.append("                        ket_cg[lane * ND + cdd] = %s_prim_aoinds[ll];" % d)
+        o.append("                        ket_cg[lane * ND + cdd] = %s_prim_aoinds[ll] - aoff;" % d)
     else:
-        o.append("                        ket_cg[lane * ND + cdd] = %s_prim_aoinds[(ll / ND) + %s_prim_count * (ll %% ND)];" % (d, d))
+        o.append("                        ket_cg[lane * ND + cdd] = %s_prim_aoinds[(ll / ND) + %s_prim_count * (ll %% ND)] - aoff;" % (d, d))
     o.append("                    }")
     o.append("                }")
     o.append("            }")
@@ -455,7 +505,7 @@ def emit_k(cls, mixed=False):
         pre = []
         for ibb in range(nb):
             for idd in range(nd):
-                pre.append("const double Dj%d_%d = mat_D_full_AO[(size_t)cgJ[%d] * naos + ket_cg[s * ND + %d]];" % (ibb, idd, ibb, idd))
+                pre.append("const double Dj%d_%d = Dm[(size_t)cgJ[%d] * nao + ket_cg[s * ND + %d]];" % (ibb, idd, ibb, idd))
         for icc in range(nc):
             for idd in range(nd):
                 tcd = icc * nd + idd
@@ -498,17 +548,29 @@ def emit_k(cls, mixed=False):
     o.append("        for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);")
     o.append("        if (lane == 0) mat_K[e0 + t] += v;")
     o.append("    }")
+    if batched:
+        o.append("    __syncthreads();")
+        o.append("    }")
     o.append("}")
     o.append("")
     lsig = _strip_restrict(sig)
     args = _argnames(sig)
-    o.append('extern "C" int vlxmi_launch_%s(%s,\n    void* stream)' % (name, ",\n    ".join(lsig)))
+    if batched:
+        o.append('extern "C" int vlxmi_launch_%s(%s,\n    uint32_t n_workgroups, void* stream)' % (name, ",\n    ".join(lsig)))
+    else:
+        o.append('extern "C" int vlxmi_launch_%s(%s,\n    void* stream)' % (name, ",\n    ".join(lsig)))
     o.append("{")
     o.append("    if (pair_inds_count_for_K_%s %% %d != 0) return VLXMI_ERR_LAYOUT;" % (ac, nac))
     o.append("    const uint32_t nblk = pair_inds_count_for_K_%s / %d;" % (ac, nac))
     o.append("    if (nblk == 0) return VLXMI_OK;")
     o.append("    if (!mat_K || !mat_D_full_AO || !boys_func_table) return VLXMI_ERR_ARG;")
-    o.append("    hipLaunchKernelGGL(%s, dim3(nblk), dim3(TILE_SIZE_K), 0, (hipStream_t)stream, %s);" % (name, ", ".join(args)))
+    if batched:
+        o.append("    if (nlist == 0) return VLXMI_OK;")
+        o.append("    if (!blk_list || !blk_mol || !mol_maxD || !mol_D_off || !mol_naos || !mol_ao_off) return VLXMI_ERR_ARG;")
+        o.append("    if (n_workgroups == 0 || n_workgroups > nlist) n_workgroups = nlist;")
+        o.append("    hipLaunchKernelGGL(%s, dim3(n_workgroups), dim3(TILE_SIZE_K), 0, (hipStream_t)stream, %s);" % (name, ", ".join(args)))
+    else:
+        o.append("    hipLaunchKernelGGL(%s, dim3(nblk), dim3(TILE_SIZE_K), 0, (hipStream_t)stream, %s);" % (name, ", ".join(args)))
     o.append("    return (int)hipGetLastError();")
     o.append("}")
     return "\n".join(o) + "\n"

@@ -57,6 +57,18 @@ def lib():
         fn = getattr(L, "vlxmi_launch_computeExchangeFock" + cls.upper() + "_mixed")
         fn.argtypes = [_ctype_of(s) for s in emit.k_signature(cls)] + [C.c_double, C.c_void_p]
         fn.restype = C.c_int
+    kb_extra = [C.c_void_p, C.c_uint32, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]
+    for mixed in ("", "_mixed"):
+        tail = [C.c_double] if mixed else []
+        for ab, cd in j_classes():
+            fn = getattr(L, "vlxmi_launch_computeCoulombFock" + (ab + cd).upper() + "_batched" + mixed)
+            fn.argtypes = [_ctype_of(s) for s in emit.j_signature(ab + cd)] + [C.c_void_p, C.c_uint32] + tail \
+                + [C.c_uint32, C.c_void_p]
+            fn.restype = C.c_int
+        for cls in k_classes():
+            fn = getattr(L, "vlxmi_launch_computeExchangeFock" + cls.upper() + "_batched" + mixed)
+            fn.argtypes = [_ctype_of(s) for s in emit.k_signature(cls)] + kb_extra + tail + [C.c_uint32, C.c_void_p]
+            fn.restype = C.c_int
     for pc in PAIR_CLASSES:
         fn = getattr(L, "vlxmi_schwarz_" + pc)
         fn.argtypes = [C.c_void_p, C.c_uint32, C.c_void_p, C.c_uint32, C.c_void_p, C.c_void_p, C.c_void_p,
@@ -87,6 +99,9 @@ def exported_symbols():
     names += ["vlxmi_launch_computeExchangeFock" + c.upper() for c in k_classes()]
     names += ["vlxmi_launch_computeCoulombFock" + (ab + cd).upper() + "_mixed" for ab, cd in j_classes()]
     names += ["vlxmi_launch_computeExchangeFock" + c.upper() + "_mixed" for c in k_classes()]
+    for mixed in ("", "_mixed"):
+        names += ["vlxmi_launch_computeCoulombFock" + (ab + cd).upper() + "_batched" + mixed for ab, cd in j_classes()]
+        names += ["vlxmi_launch_computeExchangeFock" + c.upper() + "_batched" + mixed for c in k_classes()]
     names += ["vlxmi_schwarz_" + pc for pc in PAIR_CLASSES]
     return names
 

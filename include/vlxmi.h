@@ -34,6 +34,15 @@ int vlxmi_count_k_quartets(const uint32_t* ikI, const uint32_t* ikK, uint32_t nb
                            const double* qc, const uint32_t* qc_displ, const uint32_t* qc_cnt,
                            double thr_over_maxD, unsigned long long* out, void* stream);
 
+/* Deterministic AO assembly (segmented gather-sum, no atomics):
+ * out[seg_out[s]] = sum_{t = seg_off[s]}^{seg_off[s+1]-1} values[perm[t]], in list order.
+ * Replaces the AO assembly of VeloxChem's host driver (not in the reference tree). */
+int vlxmi_segment_sum(const double* values, const uint32_t* perm, const uint32_t* seg_off,
+                      const uint32_t* seg_out, uint32_t nseg, double* out, void* stream);
+
+/* p[0..n) = 0 by a kernel (safe inside HIP-graph capture). */
+int vlxmi_fill_zero(double* p, uint64_t n, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -135,3 +135,45 @@ extern "C" int vlxmi_count_k_quartets(const uint32_t* ikI, const uint32_t* ikK, 
                        qa_cnt, qc, qc_displ, qc_cnt, thr_over_maxD, out);
     return (int)hipGetLastError();
 }
+
+// Deterministic AO assembly: out[seg_out[s]] = sum over t in [seg_off[s], seg_off[s+1])
+// of values[perm[t]], summed in list order (no atomics, one writer per output).
+__global__ void segment_sum_kernel(const double* __restrict__ values, const uint32_t* __restrict__ perm,
+                                   const uint32_t* __restrict__ seg_off, const uint32_t* __restrict__ seg_out,
+                                   uint32_t nseg, double* __restrict__ out)
+{
+    const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= nseg) return;
+    double acc = 0.0;
+    const uint32_t e = seg_off[s + 1];
+    for (uint32_t t = seg_off[s]; t < e; ++t) acc += values[perm[t]];
+    out[seg_out[s]] = acc;
+}
+
+extern "C" int vlxmi_segment_sum(const double* values, const uint32_t* perm, const uint32_t* seg_off,
+                                 const uint32_t* seg_out, uint32_t nseg, double* out, void* stream)
+{
+    if (nseg == 0) return VLXMI_OK;
+    if (!values || !perm || !seg_off || !seg_out || !out) return VLXMI_ERR_ARG;
+    hipLaunchKernelGGL(segment_sum_kernel, dim3((nseg + 255) / 256), dim3(256), 0, (hipStream_t)stream, values,
+                       perm, seg_off, seg_out, nseg, out);
+    return (int)hipGetLastError();
+}
+
+// p[0..n) = 0 with a kernel (graph-capture safe: records no memset node).
+__global__ void fill_zero_kernel(double* __restrict__ p, uint64_t n)
+{
+    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    for (uint64_t k = i; k < n; k += stride) p[k] = 0.0;
+}
+
+extern "C" int vlxmi_fill_zero(double* p, uint64_t n, void* stream)
+{
+    if (n == 0) return VLXMI_OK;
+    if (!p) return VLXMI_ERR_ARG;
+    uint64_t blocks = (n + 255) / 256;
+    if (blocks > 65536) blocks = 65536;
+    hipLaunchKernelGGL(fill_zero_kernel, dim3((unsigned)blocks), dim3(256), 0, (hipStream_t)stream, p, n);
+    return (int)hipGetLastError();
+}

@@ -85,18 +85,26 @@ class FockEngine:
 
     # ------------------------------------------------------------------ assembly
     def _build_assembly(self):
+        """Deterministic assembly plans (value index -> Cartesian AO element)."""
+        from .assemble import Assembler
         fl, d, n = self.fl, self.dev, self.fl.naos
-        self.jasm = {}
-        for pc, jl in fl.jlists.items():
+        vi, oi, off = [], [], 0
+        self.j_order = [pc for pc in P.PAIR_CLASSES if pc in fl.jlists]
+        for pc in self.j_order:
+            jl = fl.jlists[pc]
             use = ~jl.lower_diag
             f, s = jl.ao_first, jl.ao_second
-            off = (jl.first != jl.second) if jl.la == jl.lb else np.ones(jl.count, dtype=bool)
+            offd = (jl.first != jl.second) if jl.la == jl.lb else np.ones(jl.count, dtype=bool)
             sel1 = np.nonzero(use)[0]
-            sel2 = np.nonzero(use & off)[0]
-            self.jasm[pc] = (d.tensor(sel1.astype(np.int64)), d.tensor((f[sel1] * n + s[sel1]).astype(np.int64)),
-                             d.tensor(sel2.astype(np.int64)), d.tensor((s[sel2] * n + f[sel2]).astype(np.int64)))
-        self.kasm = {}
-        for pc, ik in fl.iklists.items():
+            sel2 = np.nonzero(use & offd)[0]
+            vi += [off + sel1, off + sel2]
+            oi += [f[sel1] * n + s[sel1], s[sel2] * n + f[sel2]]
+            off += jl.count
+        self.jasm = Assembler(d, np.concatenate(vi), np.concatenate(oi), n * n)
+        vi, oi, off = [], [], 0
+        self.k_order = [pc for pc in P.PAIR_CLASSES if pc in fl.iklists]
+        for pc in self.k_order:
+            ik = fl.iklists[pc]
             la, lc = ik.la, ik.lc
             na, nc = NCART[la], NCART[lc]
             ai = fl.prims[la].aoinds[ik.i % na, ik.i // na].astype(np.int64)
@@ -104,30 +112,31 @@ class FockEngine:
             if la != lc:
                 sel2 = np.arange(ik.i.size)
             else:
-                Iblk = np.repeat(ik.I, na * nc)
-                Kblk = np.repeat(ik.K, na * nc)
-                sel2 = np.nonzero(Iblk != Kblk)[0]
-            self.kasm[pc] = (d.tensor(ai * n + ak), d.tensor(sel2.astype(np.int64)),
-                             d.tensor((ak[sel2] * n + ai[sel2]).astype(np.int64)))
+                sel2 = np.nonzero(np.repeat(ik.I, na * nc) != np.repeat(ik.K, na * nc))[0]
+            vi += [off + np.arange(ik.i.size), off + sel2]
+            oi += [ai * n + ak, ak[sel2] * n + ai[sel2]]
+            off += int(ik.i.size)
+        self.kasm = Assembler(d, np.concatenate(vi), np.concatenate(oi), n * n)
+
+    def _cat(self, m, order, counts):
+        t = self.torch
+        parts = [m[pc] if pc in m else t.zeros(counts[pc], dtype=t.float64, device=self.dev.device) for pc in order]
+        return t.cat(parts) if parts else t.zeros(0, dtype=t.float64, device=self.dev.device)
 
     def assemble_J(self, matJ):
         t = self.torch
         n = self.naos
+        vals = self._cat(matJ, self.j_order, {pc: self.jl[pc]["count"] for pc in self.j_order})
         J = t.zeros(n * n, dtype=t.float64, device=self.dev.device)
-        for pc, v in matJ.items():
-            s1, i1, s2, i2 = self.jasm[pc]
-            J.index_add_(0, i1, v.index_select(0, s1))
-            J.index_add_(0, i2, v.index_select(0, s2))
+        self.jasm.run(vals, J)
         return J.view(n, n)
 
     def assemble_K(self, matK):
         t = self.torch
         n = self.naos
+        vals = self._cat(matK, self.k_order, {pc: self.ik[pc]["count"] for pc in self.k_order})
         K = t.zeros(n * n, dtype=t.float64, device=self.dev.device)
-        for pc, v in matK.items():
-            i1, s2, i2 = self.kasm[pc]
-            K.index_add_(0, i1, v)
-            K.index_add_(0, i2, v.index_select(0, s2))
+        self.kasm.run(vals, K)
         return K.view(n, n)
 
     # ------------------------------------------------------------------ per build

@@ -80,6 +80,8 @@ def lib():
     L.vlxmi_status_string.restype = C.c_char_p
     L.vlxmi_status_string.argtypes = [C.c_int]
     L.vlxmi_version.restype = C.c_int
+    L.vlxmi_segment_sum.argtypes = [C.c_void_p] * 4 + [C.c_uint32, C.c_void_p, C.c_void_p]
+    L.vlxmi_fill_zero.argtypes = [C.c_void_p, C.c_uint64, C.c_void_p]
     L.vlxmi_count_k_quartets.argtypes = [C.c_void_p, C.c_void_p, C.c_uint32] + [C.c_void_p] * 6 + [C.c_double, C.c_void_p, C.c_void_p]
     _lib = L
     return L
@@ -94,7 +96,7 @@ def check(rc: int, what: str = ""):
 def exported_symbols():
     """Names every C-ABI entry point the header declares."""
     names = ["vlxmi_version", "vlxmi_boys_table_size", "vlxmi_boys_table_fill", "vlxmi_status_string",
-             "vlxmi_boys_eval", "vlxmi_count_k_quartets"]
+             "vlxmi_boys_eval", "vlxmi_count_k_quartets", "vlxmi_segment_sum", "vlxmi_fill_zero"]
     names += ["vlxmi_launch_computeCoulombFock" + (ab + cd).upper() for ab, cd in j_classes()]
     names += ["vlxmi_launch_computeExchangeFock" + c.upper() for c in k_classes()]
     names += ["vlxmi_launch_computeCoulombFock" + (ab + cd).upper() + "_mixed" for ab, cd in j_classes()]

@@ -89,6 +89,9 @@ def main():
     ap.add_argument("--workload", default="lattice64")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--per-class-out", default=None, help="write per-class times/quartets/FLOPs (JSON)")
+    ap.add_argument("--calib-copy-mib", type=int, default=0,
+                    help="run one device copy of this many MiB first (reference kernel for HBM counters)")
+    ap.add_argument("--precision", choices=("both", "fp64", "mixed"), default="both")
     a = ap.parse_args()
     if a.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py: this build runs the J/K path on one GPU only; --gpus %d is not supported "
@@ -101,6 +104,13 @@ def main():
     from vlxmi import pairs as P
     from vlxmi.fock import FockEngine
 
+    if a.calib_copy_mib:
+        n = a.calib_copy_mib * (1 << 20) // 8
+        src = torch.ones(n, dtype=torch.float64, device="cuda")
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        del src, dst
     mol, D = configs.make_workload(a.workload)
     t0 = time.time()
     eng = FockEngine(mol)
@@ -126,8 +136,14 @@ def main():
             per.setdefault(name, []).append(e0.elapsed_time(e1) * 1e-3)
         return J, K, dt, per
 
-    J64, K64, dt64, per64 = timed("fp64")
-    J, K, dt, per = timed("mixed")
+    if a.precision == "mixed":
+        J64, K64, dt64, per64 = timed("mixed")
+    else:
+        J64, K64, dt64, per64 = timed("fp64")
+    if a.precision == "fp64":
+        J, K, dt, per = J64, K64, dt64, per64
+    else:
+        J, K, dt, per = timed("mixed")
     kern_total = sum(sum(v) for v in per.values()) / a.steps
     EJ64, EK64 = eng.energies(Dd, J64, K64)
     EJ, EK = eng.energies(Dd, J, K)

@@ -63,19 +63,106 @@ def quartet_prologue(L, ty_geo="double", omega=False, ind="    "):
         o.append("if (omega != 0.0) { double dp = d2; "
                  + " ".join("Fb[%d] *= dp; dp *= d2;" % m for m in range(1, L + 1)) + " }")
     for m in range(L + 1):
-        o.append("const %s F%d = %s(Fb[%d] * Lambda);" % (ty_geo, m, ty_geo, m))
-    o.append("const %s WP_x = %s(eta * inv_S4 * PQ_x), WP_y = %s(eta * inv_S4 * PQ_y), WP_z = %s(eta * inv_S4 * PQ_z);"
+        o.append("%s F%d = %s(Fb[%d] * Lambda);" % (ty_geo, m, ty_geo, m))
+    o.append("%s WP_x = %s(eta * inv_S4 * PQ_x), WP_y = %s(eta * inv_S4 * PQ_y), WP_z = %s(eta * inv_S4 * PQ_z);"
              % (ty_geo, ty_geo, ty_geo, ty_geo))
-    o.append("const %s WQ_x = %s(-zeta * inv_S4 * PQ_x), WQ_y = %s(-zeta * inv_S4 * PQ_y), WQ_z = %s(-zeta * inv_S4 * PQ_z);"
+    o.append("%s WQ_x = %s(-zeta * inv_S4 * PQ_x), WQ_y = %s(-zeta * inv_S4 * PQ_y), WQ_z = %s(-zeta * inv_S4 * PQ_z);"
              % (ty_geo, ty_geo, ty_geo, ty_geo))
-    o.append("const %s rz = %s(eta * inv_S4), re = %s(zeta * inv_S4);" % (ty_geo, ty_geo, ty_geo))
-    o.append("const %s h1 = %s(0.5 * inv_S1), h2 = %s(0.5 * inv_eta), h4 = %s(0.5 * inv_S4);"
+    o.append("%s rz = %s(eta * inv_S4), re = %s(zeta * inv_S4);" % (ty_geo, ty_geo, ty_geo))
+    o.append("%s h1 = %s(0.5 * inv_S1), h2 = %s(0.5 * inv_eta), h4 = %s(0.5 * inv_S4);"
              % (ty_geo, ty_geo, ty_geo, ty_geo))
-    o.append("const %s gPA_x = %s(PA_x), gPA_y = %s(PA_y), gPA_z = %s(PA_z);" % ((ty_geo,) * 4))
-    o.append("const %s gQC_x = %s(QC_x), gQC_y = %s(QC_y), gQC_z = %s(QC_z);" % ((ty_geo,) * 4))
-    o.append("const %s gAB_x = %s(AB_x), gAB_y = %s(AB_y), gAB_z = %s(AB_z);" % ((ty_geo,) * 4))
-    o.append("const %s gCD_x = %s(CD_x), gCD_y = %s(CD_y), gCD_z = %s(CD_z);" % ((ty_geo,) * 4))
+    o.append("%s gPA_x = %s(PA_x), gPA_y = %s(PA_y), gPA_z = %s(PA_z);" % ((ty_geo,) * 4))
+    o.append("%s gQC_x = %s(QC_x), gQC_y = %s(QC_y), gQC_z = %s(QC_z);" % ((ty_geo,) * 4))
+    o.append("%s gAB_x = %s(AB_x), gAB_y = %s(AB_y), gAB_z = %s(AB_z);" % ((ty_geo,) * 4))
+    o.append("%s gCD_x = %s(CD_x), gCD_y = %s(CD_y), gCD_z = %s(CD_z);" % ((ty_geo,) * 4))
     return [ind + x for x in o]
+
+
+GEO_VARS = ["WP_x", "WP_y", "WP_z", "WQ_x", "WQ_y", "WQ_z", "rz", "re", "h1", "h2", "h4",
+            "gPA_x", "gPA_y", "gPA_z", "gQC_x", "gQC_y", "gQC_z", "gAB_x", "gAB_y", "gAB_z",
+            "gCD_x", "gCD_y", "gCD_z"]
+
+
+def launder(L):
+    """Empty volatile asm statements that make the quartet inputs opaque, so the
+    compiler cannot merge (CSE / hoist) the recomputed intermediates of two
+    slices and stretch their live ranges across the whole quartet."""
+    names = ["F%d" % m for m in range(L + 1)] + GEO_VARS
+    out = []
+    for i in range(0, len(names), 10):
+        grp = names[i:i + 10]
+        out.append('asm volatile("" : %s);' % ", ".join('"+v"(%s)' % n for n in grp))
+    return out
+
+
+def _live_estimate(la, lb, lc, ld, cgroup):
+    """Largest number of simultaneously live intermediates of one slice (the
+    ket-first-index components in cgroup), in emission order."""
+    import re as _re
+    g = QuartetDAG(la, lb, lc, ld, rule_vrr="first", rule_hrr="first")
+    for ic in cgroup:
+        cc = tri(lc)[ic]
+        for aa in tri(la):
+            for bb in tri(lb):
+                for dd in tri(ld):
+                    g.lines.append("USE %s;" % g.X(aa, bb, cc, dd))
+    last = {}
+    for i, ln in enumerate(g.lines):
+        rhs = ln.split("=", 1)[1] if ln.startswith("const") else ln
+        for u in _re.findall(r"\bq\d+\b", rhs):
+            last[u] = i
+    live = mx = 0
+    ends = {}
+    for v, i in last.items():
+        ends.setdefault(i, []).append(v)
+    for i, ln in enumerate(g.lines):
+        m = _re.match(r"const \w+ (q\d+) = ", ln)
+        if m and m.group(1) in last:
+            live += 1
+            mx = max(mx, live)
+        live -= sum(1 for v in ends.get(i, []))
+    return mx, g.nflop
+
+
+_SLICE_CACHE = {}
+
+
+REG_BUDGET = 240          # doubles per lane (512 VGPR+AGPR minus addresses/indices)
+
+
+def persistent_doubles(kind, la, lb, lc, ld):
+    """Per-lane values live across a whole quartet besides the recursion
+    intermediates: accumulators, per-component thresholds (and K's S_ab, D_bd),
+    prefactors and Boys values."""
+    na, nb, nc, nd = [(l + 1) * (l + 2) // 2 for l in (la, lb, lc, ld)]
+    base = 40 + (la + lb + lc + ld)
+    if kind == "J":
+        return 2 * na * nb + base
+    return na * nc + 2 * na * nb + nb * nd + base
+
+
+def plan_slices(la, lb, lc, ld, kind="J", budget=REG_BUDGET):
+    """Partition of the ket-first-index components into passes: the fewest
+    passes whose largest live set plus the per-lane persistent values fits
+    ``budget`` doubles; one component per pass when none fits."""
+    key = (la, lb, lc, ld, kind, budget)
+    if key in _SLICE_CACHE:
+        return _SLICE_CACHE[key]
+    nc = (lc + 1) * (lc + 2) // 2
+    nd = (ld + 1) * (ld + 2) // 2
+    pers = persistent_doubles(kind, la, lb, lc, ld)
+    best = [[i] for i in range(nc)]
+    for nsl in range(1, nc + 1):
+        if nc % nsl:
+            continue
+        size = nc // nsl
+        groups = [list(range(k * size, (k + 1) * size)) for k in range(nsl)]
+        need = max(_live_estimate(la, lb, lc, ld, g_)[0] for g_ in groups) + pers + 2 * size * nd
+        if need <= budget:
+            best = groups
+            break
+    _SLICE_CACHE[key] = best
+    return best
 
 
 def _dag_lines(dag):
@@ -269,20 +356,24 @@ def emit_j(cls, mixed=False, batched=False):
     o.append("                const double CD_x = R[9], CD_y = R[10], CD_z = R[11];")
     def jbody(tyq, ind):
         lines = quartet_prologue(L, ty_geo=tyq, ind=ind)
-        dag = QuartetDAG(la, lb, lc, ld, ty=tyq, rule_vrr="first", rule_hrr="first")
-        dig = []
-        for ic, cc in enumerate(tri(lc)):
-            for idd, dd in enumerate(tri(ld)):
-                tcd = ic * nd + idd
-                dig.append("const double Dp%d = R[%d], Qd%d = R[%d];" % (tcd, 12 + tcd, tcd, 12 + ncd + tcd))
-                for ia, aa in enumerate(tri(la)):
-                    for ib, bbb in enumerate(tri(lb)):
-                        tab = ia * nb + ib
-                        x = dag.X(aa, bbb, cc, dd)
-                        xv = x if tyq == "double" else "double(%s)" % x
-                        dag.lines.append("acc[%d] += (Qd%d > tau[%d]) ? (%s * Dp%d) : 0.0;" % (tab, tcd, tab, xv, tcd))
-        lines += [ind + x for x in dig]
-        lines += [ind + x for x in _dag_lines(dag)]
+        for isl, group in enumerate(plan_slices(la, lb, lc, ld, "J")):
+            if isl:
+                lines += [ind + x for x in launder(L)]
+            dag = QuartetDAG(la, lb, lc, ld, ty=tyq, rule_vrr="first", rule_hrr="first", prefix="q%d_" % isl)
+            dig = []
+            for ic in group:
+                cc = tri(lc)[ic]
+                for idd, dd in enumerate(tri(ld)):
+                    tcd = ic * nd + idd
+                    dig.append("const double Dp%d = R[%d], Qd%d = R[%d];" % (tcd, 12 + tcd, tcd, 12 + ncd + tcd))
+                    for ia, aa in enumerate(tri(la)):
+                        for ib, bbb in enumerate(tri(lb)):
+                            tab = ia * nb + ib
+                            x = dag.X(aa, bbb, cc, dd)
+                            xv = x if tyq == "double" else "double(%s)" % x
+                            dag.lines.append("acc[%d] += (Qd%d > tau[%d]) ? (%s * Dp%d) : 0.0;" % (tab, tcd, tab, xv, tcd))
+            lines += [ind + x for x in dig]
+            lines += [ind + x for x in _dag_lines(dag)]
         return lines
 
     if mixed:
@@ -506,26 +597,32 @@ def emit_k(cls, mixed=False, batched=False):
         for ibb in range(nb):
             for idd in range(nd):
                 pre.append("const double Dj%d_%d = Dm[(size_t)cgJ[%d] * nao + ket_cg[s * ND + %d]];" % (ibb, idd, ibb, idd))
-        for icc in range(nc):
-            for idd in range(nd):
-                tcd = icc * nd + idd
-                pre.append("const double Sc%d = R[%d], Qk%d = R[%d];" % (tcd, 12 + tcd, tcd, 12 + ncd + tcd))
-        dag = QuartetDAG(la, lb, lc, ld, ty=tyq, rule_vrr="first", rule_hrr="first")
-        # digestion: acc[a,c] += Sab * sum_{b,d} mask * X * Sc * D  (FP64 accumulation)
-        for ic, cc in enumerate(tri(lc)):
-            for ia, aa in enumerate(tri(la)):
-                tac = ia * nc + ic
-                for ib, bbx in enumerate(tri(lb)):
-                    tab = ia * nb + ib
-                    terms = []
-                    for idd, dd in enumerate(tri(ld)):
-                        tcd = ic * nd + idd
-                        x = dag.X(aa, bbx, cc, dd)
-                        xv = x if tyq == "double" else "double(%s)" % x
-                        terms.append("((Qk%d > tau[%d]) ? (%s * Sc%d * Dj%d_%d) : 0.0)" % (tcd, tab, xv, tcd, ib, idd))
-                    dag.lines.append("acc[%d] += Sab[%d] * (%s);" % (tac, tab, " + ".join(terms)))
         lines += [ind + x for x in pre]
-        lines += [ind + x for x in _dag_lines(dag)]
+        # digestion: acc[a,c] += Sab * sum_{b,d} mask * X * Sc * D  (FP64 accumulation)
+        # K is not sliced: its digestion operands (D_bd gathers, S_ab, thresholds)
+        # stay live across passes, and measured scratch grew with slicing
+        for isl, group in enumerate([list(range(nc))]):
+            if isl:
+                lines += [ind + x for x in launder(L)]
+            for icc in group:
+                for idd in range(nd):
+                    tcd = icc * nd + idd
+                    lines.append(ind + "const double Sc%d = R[%d], Qk%d = R[%d];" % (tcd, 12 + tcd, tcd, 12 + ncd + tcd))
+            dag = QuartetDAG(la, lb, lc, ld, ty=tyq, rule_vrr="first", rule_hrr="first", prefix="q%d_" % isl)
+            for ic in group:
+                cc = tri(lc)[ic]
+                for ia, aa in enumerate(tri(la)):
+                    tac = ia * nc + ic
+                    for ib, bbx in enumerate(tri(lb)):
+                        tab = ia * nb + ib
+                        terms = []
+                        for idd, dd in enumerate(tri(ld)):
+                            tcd = ic * nd + idd
+                            x = dag.X(aa, bbx, cc, dd)
+                            xv = x if tyq == "double" else "double(%s)" % x
+                            terms.append("((Qk%d > tau[%d]) ? (%s * Sc%d * Dj%d_%d) : 0.0)" % (tcd, tab, xv, tcd, ib, idd))
+                        dag.lines.append("acc[%d] += Sab[%d] * (%s);" % (tac, tab, " + ".join(terms)))
+            lines += [ind + x for x in _dag_lines(dag)]
         return lines
 
     if mixed:

@@ -103,7 +103,7 @@ class QuartetDAG:
         if k == 1:
             return base
         self.mults.add((base, k))
-        return "%s_%d" % (base, k)
+        return "%sm_%s_%d" % (self.prefix, base, k)
 
     # ---------------------------------------------------------------- VRR
     def V(self, e, f, m):
@@ -179,7 +179,7 @@ class QuartetDAG:
     def mult_decls(self):
         out = []
         for base, k in sorted(self.mults):
-            out.append("const %s %s_%d = %s(%d) * %s;" % (self.ty, base, k, self.ty, k, base))
+            out.append("const %s %sm_%s_%d = %s(%d) * %s;" % (self.ty, self.prefix, base, k, self.ty, k, base))
         return out
 
 

@@ -61,3 +61,44 @@ def main():
 
 if __name__ == "__main__":
     main()
+
+
+# --------------------------------------------------------------------------
+# counter collections (rocprofv3 --pmc ... --output-format csv)
+# --------------------------------------------------------------------------
+
+def counters_from(path: str):
+    """{(dispatch id, kernel): {counter: value, "_t": seconds}} from run_counter_collection.csv."""
+    out = {}
+    with open(path) as f:
+        for r in csv.DictReader(f):
+            key = (int(r["Dispatch_Id"]), _short(r["Kernel_Name"]))
+            d = out.setdefault(key, {})
+            d[r["Counter_Name"]] = d.get(r["Counter_Name"], 0.0) + float(r["Counter_Value"])
+            d["_t"] = (int(r["End_Timestamp"]) - int(r["Start_Timestamp"])) * 1e-9
+    return out
+
+
+def flops_f64(c):
+    """Executed FP64 FLOPs of a dispatch (wave64 instructions x 64 lanes; FMA = 2)."""
+    return 64.0 * (c.get("SQ_INSTS_VALU_ADD_F64", 0) + c.get("SQ_INSTS_VALU_MUL_F64", 0)
+                   + 2 * c.get("SQ_INSTS_VALU_FMA_F64", 0) + c.get("SQ_INSTS_VALU_TRANS_F64", 0))
+
+
+def flops_f32(c):
+    return 64.0 * (c.get("SQ_INSTS_VALU_ADD_F32", 0) + c.get("SQ_INSTS_VALU_MUL_F32", 0)
+                   + 2 * c.get("SQ_INSTS_VALU_FMA_F32", 0) + c.get("SQ_INSTS_VALU_TRANS_F32", 0))
+
+
+def copy_calibration(fetch_path, write_path, copy_bytes):
+    """Bytes per FETCH_SIZE / WRITE_SIZE unit measured on the device copy kernel
+    (the largest elementwise copy dispatch) of ``copy_bytes`` bytes."""
+    fe = counters_from(fetch_path)
+    wr = counters_from(write_path)
+
+    def pick(d, cname):
+        cands = [(v.get(cname, 0.0), k) for k, v in d.items() if "copy" in k[1].lower()]
+        return max(cands)[0] if cands else None
+
+    f, w = pick(fe, "FETCH_SIZE"), pick(wr, "WRITE_SIZE")
+    return (copy_bytes / f if f else None), (copy_bytes / w if w else None)

@@ -140,7 +140,6 @@ def bench_batch(a):
         engs[m].build(dens[m])
     torch.cuda.synchronize()
     t1 = (time.perf_counter() - t0) / nsub
-    frac = sum(mols[m][0].natoms for m in range(nsub)) / float(sum(m.natoms for m, _ in mols)) * nmol / nsub
     out["single_molecule_path"] = {"molecules_timed": nsub, "s_per_molecule": t1}
     out["value"] = out["mixed"]["builds_per_s"]
     out["speedup_batched_fp64_vs_single"] = out["fp64"]["builds_per_s"] * t1

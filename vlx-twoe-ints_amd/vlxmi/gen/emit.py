@@ -138,6 +138,8 @@ def persistent_doubles(kind, la, lb, lc, ld):
     base = 40 + (la + lb + lc + ld)
     if kind == "J":
         return 2 * na * nb + base
+    if kind == "Kheavy":
+        return na * nc + nb * nd + base
     return na * nc + 2 * na * nb + nb * nd + base
 
 
@@ -345,18 +347,10 @@ def emit_j(cls, mixed=False, batched=False):
     o.append("        }")
     o.append("        __syncthreads();")
     o.append("        const uint32_t nt = (k_end - t0 < (uint32_t)KT) ? (k_end - t0) : (uint32_t)KT;")
-    o.append("        for (uint32_t s = 0; s < nt; ++s) {")
-    o.append("            const double* R = ket + s * REC;")
-    o.append("            if (active && qmax * R[0] * (1.0 + 1e-13) <= thr) active = false;")
-    o.append("            if (!__any(active)) break;")
-    o.append("            if (active) {")
-    o.append("                const double eta = R[1], inv_eta = R[2];")
-    o.append("                const double Q_x = R[3], Q_y = R[4], Q_z = R[5];")
-    o.append("                const double QC_x = R[6], QC_y = R[7], QC_z = R[8];")
-    o.append("                const double CD_x = R[9], CD_y = R[10], CD_z = R[11];")
-    def jbody(tyq, ind):
+
+    def jbody(tyq, ind, groups):
         lines = quartet_prologue(L, ty_geo=tyq, ind=ind)
-        for isl, group in enumerate(plan_slices(la, lb, lc, ld, "J")):
+        for isl, group in enumerate(groups):
             if isl:
                 lines += [ind + x for x in launder(L)]
             dag = QuartetDAG(la, lb, lc, ld, ty=tyq, rule_vrr="first", rule_hrr="first", prefix="q%d_" % isl)
@@ -376,17 +370,34 @@ def emit_j(cls, mixed=False, batched=False):
             lines += [ind + x for x in _dag_lines(dag)]
         return lines
 
-    if mixed:
-        o.append("                const bool lo = qmax * R[0] < fp32_threshold;")
-        o.append("                if (__all(lo)) {")
-        o += jbody("float", "                    ")
-        o.append("                } else {")
-        o += jbody("double", "                    ")
-        o.append("                }")
-    else:
-        o += jbody("double", "                ")
-    o.append("            }")
-    o.append("        }")
+    def jloop(groups, act):
+        """One loop over the staged ket records evaluating the passes ``groups``."""
+        q = []
+        q.append("        for (uint32_t s = 0; s < nt; ++s) {")
+        q.append("            const double* R = ket + s * REC;")
+        q.append("            if (%s && qmax * R[0] * (1.0 + 1e-13) <= thr) %s = false;" % (act, act))
+        q.append("            if (!__any(%s)) break;" % act)
+        q.append("            if (%s) {" % act)
+        q.append("                const double eta = R[1], inv_eta = R[2];")
+        q.append("                const double Q_x = R[3], Q_y = R[4], Q_z = R[5];")
+        q.append("                const double QC_x = R[6], QC_y = R[7], QC_z = R[8];")
+        q.append("                const double CD_x = R[9], CD_y = R[10], CD_z = R[11];")
+        if mixed:
+            q.append("                const bool lo = qmax * R[0] < fp32_threshold;")
+            q.append("                if (__all(lo)) {")
+            q.extend(jbody("float", "                    ", groups))
+            q.append("                } else {")
+            q.extend(jbody("double", "                    ", groups))
+            q.append("                }")
+        else:
+            q.extend(jbody("double", "                ", groups))
+        q.append("            }")
+        q.append("        }")
+        return q
+
+    # J keeps its passes inside one ket loop (the bra-side state is small);
+    # measured faster than one loop per pass
+    o += jloop(plan_slices(la, lb, lc, ld, "J"), "active")
     o.append("    }")
     o.append("    if (own) {")
     o.append("        const uint32_t e0 = bb * NAB;")
@@ -449,6 +460,8 @@ def emit_k(cls, mixed=False, batched=False):
     L = la + lb + lc + ld
     rec = 12 + 2 * ncd
     rec += rec % 2
+    kslices = plan_slices(la, lb, lc, ld, "Kheavy")
+    heavy = len(kslices) > 1
     name = "computeExchangeFock" + cls.upper() + ("_batched" if batched else "") + ("_mixed" if mixed else "")
     bsig = ["const uint32_t* __restrict__ blk_list", "const uint32_t nlist", "const uint32_t* __restrict__ blk_mol",
             "const double* __restrict__ mol_maxD", "const unsigned long long* __restrict__ mol_D_off",
@@ -460,9 +473,12 @@ def emit_k(cls, mixed=False, batched=False):
     o.append("%s(%s)" % (name, ",\n    ".join(sig)))
     o.append("{")
     o.append("    constexpr int NA = %d, NB = %d, NC = %d, ND = %d, NAC = %d, NAB = %d, NCD = %d;" % (na, nb, nc, nd, nac, nab, ncd))
-    o.append("    constexpr int KT = %d, REC = %d;" % (KET_TILE, rec))
+    o.append("    constexpr int KT = %d, REC = %d;" % (32 if heavy else KET_TILE, rec))
     o.append("    __shared__ double ket[KT * REC];")
     o.append("    __shared__ uint32_t ket_cg[KT * ND];")
+    if heavy:
+        # per-lane thresholds and S_ab live in LDS ([component][lane]) to free registers
+        o.append("    __shared__ double ltau[NAB * TILE_SIZE_K], lSab[NAB * TILE_SIZE_K];")
     o.append("    const uint32_t lane = threadIdx.x;")
     if batched:
         o.append("    (void)%s_max_D;" % bd)
@@ -506,10 +522,16 @@ def emit_k(cls, mixed=False, batched=False):
     o.append("        double zeta = 1.0, inv_S1 = 1.0, P_x = 0.0, P_y = 0.0, P_z = 0.0;")
     o.append("        double PA_x = 0.0, PA_y = 0.0, PA_z = 0.0, AB_x = 0.0, AB_y = 0.0, AB_z = 0.0;")
     o.append("        double qij = 0.0;")
-    o.append("        double tau[NAB], Sab[NAB];")
+    if heavy:
+        o.append("        double* tau = ltau;")
+        o.append("        double* Sab = lSab;")
+        o.append("        #pragma unroll")
+        o.append("        for (int t = 0; t < NAB; ++t) { tau[t * TILE_SIZE_K + lane] = 0.0; Sab[t * TILE_SIZE_K + lane] = 0.0; }")
+    else:
+        o.append("        double tau[NAB], Sab[NAB];")
+        o.append("        #pragma unroll")
+        o.append("        for (int t = 0; t < NAB; ++t) { tau[t] = 0.0; Sab[t] = 0.0; }")
     o.append("        uint32_t cgJ[NB];")
-    o.append("        #pragma unroll")
-    o.append("        for (int t = 0; t < NAB; ++t) { tau[t] = 0.0; Sab[t] = 0.0; }")
     o.append("        #pragma unroll")
     o.append("        for (int t = 0; t < NB; ++t) cgJ[t] = 0;")
     o.append("        if (active) {")
@@ -528,8 +550,12 @@ def emit_k(cls, mixed=False, batched=False):
     o.append("                    const uint32_t pos = displ_i[ca] + jp * NB + cb;")
     o.append("                    const double q = fabs(Q_K_%s[pos]);" % ab)
     o.append("                    qij = fmax(qij, q);")
-    o.append("                    tau[ca * NB + cb] = thr_eff / q;")
-    o.append("                    Sab[ca * NB + cb] = pair_data_K_%s[pos];" % ab)
+    if heavy:
+        o.append("                    tau[(ca * NB + cb) * TILE_SIZE_K + lane] = thr_eff / q;")
+        o.append("                    Sab[(ca * NB + cb) * TILE_SIZE_K + lane] = pair_data_K_%s[pos];" % ab)
+    else:
+        o.append("                    tau[ca * NB + cb] = thr_eff / q;")
+        o.append("                    Sab[ca * NB + cb] = pair_data_K_%s[pos];" % ab)
     o.append("                }")
     o.append("            }")
     o.append("            #pragma unroll")
@@ -582,32 +608,29 @@ def emit_k(cls, mixed=False, batched=False):
     o.append("            }")
     o.append("            __syncthreads();")
     o.append("            const uint32_t nt = (nL - t0 < (uint32_t)KT) ? (nL - t0) : (uint32_t)KT;")
-    o.append("            for (uint32_t s = 0; s < nt; ++s) {")
-    o.append("                const double* R = ket + s * REC;")
-    o.append("                if (active && qij * R[0] * maxD * (1.0 + 1e-13) <= thr) active = false;")
-    o.append("                if (!__any(active)) break;")
-    o.append("                if (active) {")
-    o.append("                    const double eta = R[1], inv_eta = R[2];")
-    o.append("                    const double Q_x = R[3], Q_y = R[4], Q_z = R[5];")
-    o.append("                    const double QC_x = R[6], QC_y = R[7], QC_z = R[8];")
-    o.append("                    const double CD_x = R[9], CD_y = R[10], CD_z = R[11];")
-    def kbody(tyq, ind):
+    def kbody(tyq, ind, groups):
         lines = quartet_prologue(L, ty_geo=tyq, omega=True, ind=ind)
-        pre = []
+        TAU = (lambda t: "tau[%d * TILE_SIZE_K + lid]" % t) if heavy else (lambda t: "tau[%d]" % t)
+        SAB = (lambda t: "Sab[%d * TILE_SIZE_K + lid]" % t) if heavy else (lambda t: "Sab[%d]" % t)
+        if heavy:
+            # opaque copies of the lane index and the D row offsets: the loads of
+            # one pass can be neither hoisted out of its ket loop nor shared
+            # with another pass
+            lines += [ind + "uint32_t lid = lane;", ind + "asm volatile(\"\" : \"+v\"(lid));"]
+            for ibb in range(nb):
+                lines += [ind + "uint32_t rowJ%d = cgJ[%d];" % (ibb, ibb), ind + "asm volatile(\"\" : \"+v\"(rowJ%d));" % ibb]
         for ibb in range(nb):
             for idd in range(nd):
-                pre.append("const double Dj%d_%d = Dm[(size_t)cgJ[%d] * nao + ket_cg[s * ND + %d]];" % (ibb, idd, ibb, idd))
-        lines += [ind + x for x in pre]
+                lines.append(ind + "const double Dj%d_%d = Dm[(size_t)%s * nao + ket_cg[s * ND + %d]];"
+                             % (ibb, idd, ("rowJ%d" % ibb) if heavy else ("cgJ[%d]" % ibb), idd))
         # digestion: acc[a,c] += Sab * sum_{b,d} mask * X * Sc * D  (FP64 accumulation)
-        # K is not sliced: its digestion operands (D_bd gathers, S_ab, thresholds)
-        # stay live across passes, and measured scratch grew with slicing
-        for isl, group in enumerate([list(range(nc))]):
+        for icc in sorted(set(c_ for g_ in groups for c_ in g_)):
+            for idd in range(nd):
+                tcd = icc * nd + idd
+                lines.append(ind + "const double Sc%d = R[%d], Qk%d = R[%d];" % (tcd, 12 + tcd, tcd, 12 + ncd + tcd))
+        for isl, group in enumerate(groups):
             if isl:
                 lines += [ind + x for x in launder(L)]
-            for icc in group:
-                for idd in range(nd):
-                    tcd = icc * nd + idd
-                    lines.append(ind + "const double Sc%d = R[%d], Qk%d = R[%d];" % (tcd, 12 + tcd, tcd, 12 + ncd + tcd))
             dag = QuartetDAG(la, lb, lc, ld, ty=tyq, rule_vrr="first", rule_hrr="first", prefix="q%d_" % isl)
             for ic in group:
                 cc = tri(lc)[ic]
@@ -620,22 +643,51 @@ def emit_k(cls, mixed=False, batched=False):
                             tcd = ic * nd + idd
                             x = dag.X(aa, bbx, cc, dd)
                             xv = x if tyq == "double" else "double(%s)" % x
-                            terms.append("((Qk%d > tau[%d]) ? (%s * Sc%d * Dj%d_%d) : 0.0)" % (tcd, tab, xv, tcd, ib, idd))
-                        dag.lines.append("acc[%d] += Sab[%d] * (%s);" % (tac, tab, " + ".join(terms)))
+                            terms.append("((Qk%d > %s) ? (%s * Sc%d * Dj%d_%d) : 0.0)" % (tcd, TAU(tab), xv, tcd, ib, idd))
+                        dag.lines.append("acc[%d] += %s * (%s);" % (tac, SAB(tab), " + ".join(terms)))
             lines += [ind + x for x in _dag_lines(dag)]
         return lines
 
-    if mixed:
-        o.append("                    const bool lo = qij * R[0] * maxD < fp32_threshold;")
-        o.append("                    if (__all(lo)) {")
-        o += kbody("float", "                        ")
-        o.append("                    } else {")
-        o += kbody("double", "                        ")
-        o.append("                    }")
+    def sloop(groups, act):
+        """One loop over the staged ket records evaluating the passes ``groups``."""
+        q = []
+        q.append("            for (uint32_t s = 0; s < nt; ++s) {")
+        q.append("                const double* R = ket + s * REC;")
+        q.append("                if (%s && qij * R[0] * maxD * (1.0 + 1e-13) <= thr) %s = false;" % (act, act))
+        q.append("                if (!__any(%s)) break;" % act)
+        q.append("                if (%s) {" % act)
+        q.append("                    const double eta = R[1], inv_eta = R[2];")
+        q.append("                    const double Q_x = R[3], Q_y = R[4], Q_z = R[5];")
+        q.append("                    const double QC_x = R[6], QC_y = R[7], QC_z = R[8];")
+        q.append("                    const double CD_x = R[9], CD_y = R[10], CD_z = R[11];")
+        if mixed:
+            q.append("                    const bool lo = qij * R[0] * maxD < fp32_threshold;")
+            q.append("                    if (__all(lo)) {")
+            q.extend(kbody("float", "                        ", groups))
+            q.append("                    } else {")
+            q.extend(kbody("double", "                        ", groups))
+            q.append("                    }")
+        else:
+            q.extend(kbody("double", "                    ", groups))
+        q.append("                }")
+        q.append("            }")
+        return q
+
+    if heavy:
+        # heavy classes: one loop over the tile per pass of ket-first-index
+        # components, so that only one pass's recursion is live at a time; the
+        # screening sequence is the same in every pass, so each pass stops at
+        # the same record
+        o.append("            bool act_end = active;")
+        for isl, group in enumerate(kslices):
+            o.append("            {")
+            o.append("            bool act%d = active;" % isl)
+            o += sloop([group], "act%d" % isl)
+            o.append("            act_end = act%d;" % isl)
+            o.append("            }")
+        o.append("            active = act_end;")
     else:
-        o += kbody("double", "                    ")
-    o.append("                }")
-    o.append("            }")
+        o += sloop([list(range(nc))], "active")
     o.append("        }")
     o.append("    }")
     o.append("    #pragma unroll")

@@ -28,6 +28,14 @@ PEAK_FP64_VALU = 78.6e12     # MI355X FP64 vector, FLOP/s
 PEAK_HBM = 8.0e12            # bytes/s
 
 
+_T0 = time.time()
+
+
+def _progress(msg):
+    """One progress line on stderr (long set-up stages otherwise print nothing)."""
+    print("[bench %7.1f s] %s" % (time.time() - _T0, msg), file=sys.stderr, flush=True)
+
+
 def cpu_baseline(eng, D_cart, nq, seed=0):
     """Time the CPU oracle (FP64 C restatement, OpenMP) on a bounded sample of
     the same workload and scale to one full J+K build.
@@ -97,8 +105,14 @@ def bench_batch(a):
     dev = Device()
     t0 = time.time()
     mols = [configs.make_batch_molecule(m, hi=hi) for m in range(nmol)]
-    entries = [MolEntry(mol, dev) for mol, _ in mols]
+    _progress("batch: %d molecules generated" % nmol)
+    entries = []
+    for i, (mol, _) in enumerate(mols):
+        entries.append(MolEntry(mol, dev))
+        if (i + 1) % 64 == 0:
+            _progress("batch: %d molecule entries resident" % (i + 1))
     pack = Pack(entries, dev)
+    _progress("batch: pack built")
     dens = [D for _, D in mols]
     torch.cuda.synchronize()
     setup = time.time() - t0
@@ -110,8 +124,10 @@ def bench_batch(a):
            "setup_s": setup}
     for prec in ("fp64", "mixed"):
         be = BatchEngine(dev, precision=prec, fp32_threshold=a.fp32_threshold)
-        tune = be.autotune(pack, dens, reps=1)
+        tune = be.autotune(pack, dens, reps=1, progress=_progress)
+        _progress("batch %s: autotuned" % prec)
         replay = be.capture(pack, dens)
+        _progress("batch %s: graph captured" % prec)
         for _ in range(a.warmup):
             replay(dens)
         torch.cuda.synchronize()
@@ -129,6 +145,7 @@ def bench_batch(a):
             dEK = max(abs(0.5 * float((dev.tensor(D) * (Km - Kf)).sum())) for D, Km, Kf in zip(dens, K, K64))
             out[prec]["max_abs_dE_J"] = dEJ
             out[prec]["max_abs_dE_K"] = dEK
+        _progress("batch %s: %.2f builds/s" % (prec, out[prec]["builds_per_s"]))
     # single-molecule path (eager, one molecule at a time) on the first molecules, scaled
     nsub = min(16, nmol)
     engs = [FockEngine(mols[m][0], dev=dev) for m in range(nsub)]
@@ -140,6 +157,7 @@ def bench_batch(a):
         engs[m].build(dens[m])
     torch.cuda.synchronize()
     t1 = (time.perf_counter() - t0) / nsub
+    _progress("batch: single-molecule path timed")
     out["single_molecule_path"] = {"molecules_timed": nsub, "s_per_molecule": t1}
     out["value"] = out["mixed"]["builds_per_s"]
     out["speedup_batched_fp64_vs_single"] = out["fp64"]["builds_per_s"] * t1
@@ -168,13 +186,14 @@ def bench_serve(a):
         D = configs.density(rng, mol.nbf, int(math.ceil(mol.nbf / 4)), 0.3)
         mols[m] = mol
         dens.append(D)
+    _progress("serve: %d molecules generated" % nmol)
     res = []
     for rate in [float(x) for x in a.rates.split(",")]:
         reqs = poisson_requests(dens, a.iterations, rate, seed=7,
                                 perturb=lambda D0, m, it: configs.serve_iteration_density(D0, m, it))
         srv = FockServer(mols, dev=dev, hbm_budget=64e9, max_batch=32, precision="mixed",
                          fp32_threshold=a.fp32_threshold)
-        st = srv.serve(reqs)
+        st = srv.serve(reqs, progress=_progress)
         d = st.as_dict()
         d["rate"] = rate
         res.append(d)

@@ -14,7 +14,7 @@
 // For T < VLX_BOYS_TMAX: a 7-term Taylor expansion of F_L about the nearest
 // grid point (|T - T_k| <= DT/2), exp(-T) = exp(-T_k) * (8-term Taylor of
 // exp(-d)), then downward recursion F_{m} = (2T F_{m+1} + exp(-T)) / (2m+1).
-// For T >= VLX_BOYS_TMAX: F_0 = sqrt(pi/T)/2 and upward recursion
+// For T >= VLX_BOYS_TMAX: F_0 = sqrt(pi/T)/2 (Newton-refined 1/sqrt) and upward recursion
 // F_{m+1} = ((2m+1) F_m - exp(-T)) / (2T); there exp(-T) < 5e-18 and the
 // single-precision hardware exponential is accurate enough for it.
 #pragma once
@@ -38,6 +38,29 @@
 #define TILE_SIZE_K 64
 
 namespace vlxmi {
+
+// 1/x and 1/sqrt(x) for positive, normal x well inside the double range (the
+// Gaussian exponent sums and Boys arguments these are used on): the hardware
+// estimate refined by two Newton steps, each of which squares the relative
+// error, so the result is within a few ulp; no denormal/overflow scaling.
+__device__ __forceinline__ double rcp_nr(double x)
+{
+    double r = __builtin_amdgcn_rcp(x);
+    double e = fma(-x, r, 1.0);
+    r = fma(r, e, r);
+    e = fma(-x, r, 1.0);
+    return fma(r, e, r);
+}
+
+__device__ __forceinline__ double rsq_nr(double x)
+{
+    double y = __builtin_amdgcn_rsq(x);
+    const double h = 0.5 * x;
+    double t = fma(-h * y, y, 0.5);
+    y = fma(y, t, y);
+    t = fma(-h * y, y, 0.5);
+    return fma(y, t, y);
+}
 
 template <int L>
 __device__ __forceinline__ void boys(double T, const double* __restrict__ table, double* F)
@@ -71,8 +94,8 @@ __device__ __forceinline__ void boys(double T, const double* __restrict__ table,
             for (int m = L - 1; m >= 0; --m) F[m] = fma(T2, F[m + 1], et) * (1.0 / (2.0 * m + 1.0));
         }
     } else {
-        const double inv_T = 1.0 / T;
-        F[0] = 0.5 * sqrt(MATH_CONST_PI * inv_T);
+        const double inv_T = rcp_nr(T);
+        F[0] = (0.5 * 1.77245385090551602730) * rsq_nr(T);   // sqrt(pi/T) / 2
         if (L > 0) {
             const double et = (double)__expf(-(float)T);
             const double h = 0.5 * inv_T;

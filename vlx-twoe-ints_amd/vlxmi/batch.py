@@ -443,9 +443,11 @@ class BatchEngine:
         return replay
 
     # ------------------------------------------------------------------ autotuner
-    def autotune(self, pack: Pack, densities, reps=2, wg_choices=(256, 512, 1024, 2048, 0), verbose=False):
+    def autotune(self, pack: Pack, densities, reps=2, wg_choices=(256, 512, 1024, 2048, 0), verbose=False,
+                 progress=None):
         """Per class: persistent grid size x work-item order (natural or
-        costliest-first), timed on this pack; the fastest is kept in pack.tuning."""
+        costliest-first), timed on this pack; the fastest is kept in pack.tuning.
+        ``progress(msg)`` is called after each candidate round."""
         t = self.torch
         self.prepare(pack, densities)
         pack.tuning = {}
@@ -476,6 +478,8 @@ class BatchEngine:
                         dt = e0.elapsed_time(e1)
                         if dt < best[key][1]:
                             best[key] = (pack.tuning[key], dt)
+            if progress is not None:
+                progress("autotune: candidate round %d of %d" % (i + 1, nmax))
             self.timer = None
         pack.tuning = {k: v[0] for k, v in best.items() if v[0] is not None}
         summary = {}

@@ -34,6 +34,9 @@ LV = {"s": 0, "p": 1, "d": 2}
 NC = {"s": 1, "p": 3, "d": 6}
 
 KET_TILE = 64
+# ket record (LDS, doubles): 0 screening bound, 1 eta, 2 1/eta, 3-5 Q, 6-8 QC,
+# 9-11 CD, 12 sqrt(eta), then the per-component values from KREC0
+KREC0 = 13
 
 
 def _types_in(cls):
@@ -45,15 +48,18 @@ def _types_in(cls):
 # --------------------------------------------------------------------------
 
 def quartet_prologue(L, ty_geo="double", omega=False, ind="    "):
-    """Quartet prefactors.  Expects in scope: zeta, inv_S1, P_x.., PA_x.., AB_x..
-    (bra), eta, inv_eta, Q_x.., QC_x.., CD_x.. (ket), boys tables."""
+    """Quartet prefactors.  Expects in scope: zeta, sq_zeta = sqrt(zeta), inv_S1,
+    P_x.., PA_x.., AB_x.. (bra), eta, sq_eta = sqrt(eta), inv_eta, Q_x.., QC_x..,
+    CD_x.. (ket), boys tables.  One Newton-refined 1/sqrt(zeta + eta) gives both
+    1/(zeta + eta) and Lambda = sqrt(4 rho / pi) = (2/sqrt(pi)) sqrt(zeta eta) / sqrt(zeta + eta)."""
     o = []
     o.append("const double S4 = zeta + eta;")
-    o.append("const double inv_S4 = 1.0 / S4;")
+    o.append("const double rS4 = vlxmi::rsq_nr(S4);")
+    o.append("const double inv_S4 = rS4 * rS4;")
     o.append("const double rho = zeta * eta * inv_S4;")
     o.append("const double PQ_x = Q_x - P_x, PQ_y = Q_y - P_y, PQ_z = Q_z - P_z;")
     o.append("double Tb = rho * (PQ_x * PQ_x + PQ_y * PQ_y + PQ_z * PQ_z);")
-    o.append("double Lambda = sqrt(4.0 * rho * MATH_CONST_INV_PI);")
+    o.append("double Lambda = (1.12837916709551257390 * rS4) * (sq_zeta * sq_eta);")
     if omega:
         o.append("double d2 = 1.0;")
         o.append("if (omega != 0.0) { d2 = omega * omega / (rho + omega * omega); Tb *= d2; Lambda *= sqrt(d2); }")
@@ -176,6 +182,51 @@ def _dag_lines(dag):
     return out
 
 
+def ket_loops(ind, act, screen, bound, header, body, mixed, lo):
+    """Loop(s) over the staged ket records s = 0..nt-1 of one tile.
+
+    ``act`` is the lane's live flag, cleared by the screening test ``screen``
+    (the loops end when no lane is live); ``header`` reads the ket record;
+    ``body(ty, ind)`` is the quartet code with the recursion in type ``ty``.
+    Mixed precision: a quartet runs its recursion in FP32 when ``bound`` <
+    fp32_threshold on every live lane of the wavefront.  The bound is
+    non-increasing along the ket list, so the tile is one FP64 loop up to the
+    first record where that holds and one FP32 loop after it; the wave-uniform
+    flag ``lo`` carries the switch to later tiles.  Each loop then holds one
+    recursion body only."""
+    def loop(ty, check_lo):
+        q = [ind + "for (; s < nt; ++s) {",
+             ind + "    const double* R = ket + s * REC;",
+             ind + "    if (%s && %s) %s = false;" % (act, screen, act),
+             ind + "    if (!__any(%s)) break;" % act]
+        if check_lo:
+            q.append(ind + "    if (__all(!%s || %s < fp32_threshold)) { %s = true; break; }" % (act, bound, lo))
+        q.append(ind + "    if (%s) {" % act)
+        q += [ind + "        " + x for x in header]
+        q += body(ty, ind + "        ")
+        q += [ind + "    }", ind + "}"]
+        return q
+
+    o = [ind + "{", ind + "uint32_t s = 0;"]
+    if mixed:
+        o.append(ind + "if (!%s) {" % lo)
+        o += ["    " + x for x in loop("double", True)]
+        o.append(ind + "}")
+        o.append(ind + "if (%s) {" % lo)
+        o += ["    " + x for x in loop("float", False)]
+        o.append(ind + "}")
+    else:
+        o += loop("double", False)
+    o.append(ind + "}")
+    return o
+
+
+KET_HEADER = ["const double eta = R[1], inv_eta = R[2], sq_eta = R[12];",
+              "const double Q_x = R[3], Q_y = R[4], Q_z = R[5];",
+              "const double QC_x = R[6], QC_y = R[7], QC_z = R[8];",
+              "const double CD_x = R[9], CD_y = R[10], CD_z = R[11];"]
+
+
 def j_signature(cls):
     a, b, c, d = cls
     ab, cd = a + b, c + d
@@ -253,7 +304,7 @@ def emit_j(cls, mixed=False, batched=False):
     ab, cd = a + b, c + d
     nab, ncd = na * nb, nc * nd
     L = la + lb + lc + ld
-    rec = 12 + 2 * ncd
+    rec = KREC0 + 2 * ncd
     rec += rec % 2
     name = "computeCoulombFock" + cls.upper() + ("_batched" if batched else "") + ("_mixed" if mixed else "")
     sig = j_signature(cls) + (["const uint32_t* __restrict__ work", "const uint32_t nwork"] if batched else []) \
@@ -284,7 +335,7 @@ def emit_j(cls, mixed=False, batched=False):
         o.append("    const bool own = bb < nbra;")
     o.append("    const double thr = eri_threshold;")
     o.append("    bool active = own;")
-    o.append("    double zeta = 1.0, inv_S1 = 1.0, P_x = 0.0, P_y = 0.0, P_z = 0.0;")
+    o.append("    double zeta = 1.0, sq_zeta = 1.0, inv_S1 = 1.0, P_x = 0.0, P_y = 0.0, P_z = 0.0;")
     o.append("    double PA_x = 0.0, PA_y = 0.0, PA_z = 0.0, AB_x = 0.0, AB_y = 0.0, AB_z = 0.0;")
     o.append("    double qmax = 0.0;")
     o.append("    double tau[NAB];")
@@ -299,7 +350,7 @@ def emit_j(cls, mixed=False, batched=False):
     o.append("        const double A_x = %s_prim_info[I + %s_prim_count * 2], A_y = %s_prim_info[I + %s_prim_count * 3], A_z = %s_prim_info[I + %s_prim_count * 4];" % ((a,) * 6))
     o.append("        const double aj = %s_prim_info[J];" % b)
     o.append("        const double B_x = %s_prim_info[J + %s_prim_count * 2], B_y = %s_prim_info[J + %s_prim_count * 3], B_z = %s_prim_info[J + %s_prim_count * 4];" % ((b,) * 6))
-    o.append("        zeta = ai + aj; inv_S1 = 1.0 / zeta;")
+    o.append("        zeta = ai + aj; inv_S1 = 1.0 / zeta; sq_zeta = sqrt(zeta);")
     o.append("        P_x = (ai * A_x + aj * B_x) * inv_S1; P_y = (ai * A_y + aj * B_y) * inv_S1; P_z = (ai * A_z + aj * B_z) * inv_S1;")
     o.append("        PA_x = (aj * inv_S1) * (B_x - A_x); PA_y = (aj * inv_S1) * (B_y - A_y); PA_z = (aj * inv_S1) * (B_z - A_z);")
     o.append("        AB_x = A_x - B_x; AB_y = A_y - B_y; AB_z = A_z - B_z;")
@@ -310,6 +361,8 @@ def emit_j(cls, mixed=False, batched=False):
     o.append("            tau[t] = thr / q;")
     o.append("        }")
     o.append("    }")
+    if mixed:
+        o.append("    bool lo_mode = false;")
     o.append("    for (uint32_t t0 = k_begin; t0 < k_end; t0 += KT) {")
     o.append("        if (!__any(active)) break;")
     o.append("        __syncthreads();")
@@ -325,7 +378,7 @@ def emit_j(cls, mixed=False, batched=False):
     o.append("                const double al = %s_prim_info[l];" % d)
     o.append("                const double D_x = %s_prim_info[l + %s_prim_count * 2], D_y = %s_prim_info[l + %s_prim_count * 3], D_z = %s_prim_info[l + %s_prim_count * 4];" % ((d,) * 6))
     o.append("                const double et = ak + al, iet = 1.0 / et;")
-    o.append("                R[1] = et; R[2] = iet;")
+    o.append("                R[1] = et; R[2] = iet; R[12] = sqrt(et);")
     o.append("                R[3] = (ak * C_x + al * D_x) * iet; R[4] = (ak * C_y + al * D_y) * iet; R[5] = (ak * C_z + al * D_z) * iet;")
     o.append("                R[6] = (al * iet) * (D_x - C_x); R[7] = (al * iet) * (D_y - C_y); R[8] = (al * iet) * (D_z - C_z);")
     o.append("                R[9] = C_x - D_x; R[10] = C_y - D_y; R[11] = C_z - D_z;")
@@ -337,9 +390,9 @@ def emit_j(cls, mixed=False, batched=False):
         o.append("                    const double w = (%s_first_inds[f0 + t] != %s_second_inds[f0 + t]) ? 2.0 : 1.0;" % (cd, cd))
     else:
         o.append("                    const double w = 2.0;")
-    o.append("                    R[12 + t] = %s_pair_data[f0 + t] * Dkl * w;" % cd)
+    o.append("                    R[%d + t] = %s_pair_data[f0 + t] * Dkl * w;" % (KREC0, cd))
     o.append("                    const double qd = fabs(%s_mat_Q[f0 + t] * Dkl);" % cd)
-    o.append("                    R[12 + NCD + t] = qd;")
+    o.append("                    R[%d + NCD + t] = qd;" % KREC0)
     o.append("                    beta = fmax(beta, qd);")
     o.append("                }")
     o.append("                R[0] = beta;")
@@ -359,7 +412,7 @@ def emit_j(cls, mixed=False, batched=False):
                 cc = tri(lc)[ic]
                 for idd, dd in enumerate(tri(ld)):
                     tcd = ic * nd + idd
-                    dig.append("const double Dp%d = R[%d], Qd%d = R[%d];" % (tcd, 12 + tcd, tcd, 12 + ncd + tcd))
+                    dig.append("const double Dp%d = R[%d], Qd%d = R[%d];" % (tcd, KREC0 + tcd, tcd, KREC0 + ncd + tcd))
                     for ia, aa in enumerate(tri(la)):
                         for ib, bbb in enumerate(tri(lb)):
                             tab = ia * nb + ib
@@ -370,34 +423,11 @@ def emit_j(cls, mixed=False, batched=False):
             lines += [ind + x for x in _dag_lines(dag)]
         return lines
 
-    def jloop(groups, act):
-        """One loop over the staged ket records evaluating the passes ``groups``."""
-        q = []
-        q.append("        for (uint32_t s = 0; s < nt; ++s) {")
-        q.append("            const double* R = ket + s * REC;")
-        q.append("            if (%s && qmax * R[0] * (1.0 + 1e-13) <= thr) %s = false;" % (act, act))
-        q.append("            if (!__any(%s)) break;" % act)
-        q.append("            if (%s) {" % act)
-        q.append("                const double eta = R[1], inv_eta = R[2];")
-        q.append("                const double Q_x = R[3], Q_y = R[4], Q_z = R[5];")
-        q.append("                const double QC_x = R[6], QC_y = R[7], QC_z = R[8];")
-        q.append("                const double CD_x = R[9], CD_y = R[10], CD_z = R[11];")
-        if mixed:
-            q.append("                const bool lo = qmax * R[0] < fp32_threshold;")
-            q.append("                if (__all(lo)) {")
-            q.extend(jbody("float", "                    ", groups))
-            q.append("                } else {")
-            q.extend(jbody("double", "                    ", groups))
-            q.append("                }")
-        else:
-            q.extend(jbody("double", "                ", groups))
-        q.append("            }")
-        q.append("        }")
-        return q
-
     # J keeps its passes inside one ket loop (the bra-side state is small);
     # measured faster than one loop per pass
-    o += jloop(plan_slices(la, lb, lc, ld, "J"), "active")
+    jsl = plan_slices(la, lb, lc, ld, "J")
+    o += ket_loops("        ", "active", "qmax * R[0] * (1.0 + 1e-13) <= thr", "qmax * R[0]", KET_HEADER,
+                   lambda ty, ind: jbody(ty, ind, jsl), mixed, "lo_mode")
     o.append("    }")
     o.append("    if (own) {")
     o.append("        const uint32_t e0 = bb * NAB;")
@@ -458,7 +488,7 @@ def emit_k(cls, mixed=False, batched=False):
     ac, ab, cd, bd = a + c, a + b, c + d, b + d
     nac, nab, ncd = na * nc, na * nb, nc * nd
     L = la + lb + lc + ld
-    rec = 12 + 2 * ncd
+    rec = KREC0 + 2 * ncd
     rec += rec % 2
     kslices = plan_slices(la, lb, lc, ld, "Kheavy")
     heavy = len(kslices) > 1
@@ -519,7 +549,7 @@ def emit_k(cls, mixed=False, batched=False):
     o.append("    for (uint32_t jc = 0; jc < nJ; jc += TILE_SIZE_K) {")
     o.append("        const uint32_t jp = jc + lane;")
     o.append("        bool active = jp < nJ;")
-    o.append("        double zeta = 1.0, inv_S1 = 1.0, P_x = 0.0, P_y = 0.0, P_z = 0.0;")
+    o.append("        double zeta = 1.0, sq_zeta = 1.0, inv_S1 = 1.0, P_x = 0.0, P_y = 0.0, P_z = 0.0;")
     o.append("        double PA_x = 0.0, PA_y = 0.0, PA_z = 0.0, AB_x = 0.0, AB_y = 0.0, AB_z = 0.0;")
     o.append("        double qij = 0.0;")
     if heavy:
@@ -539,7 +569,7 @@ def emit_k(cls, mixed=False, batched=False):
     o.append("            const uint32_t J = jcomb / NB;")
     o.append("            const double aj = %s_prim_info[J];" % b)
     o.append("            const double B_x = %s_prim_info[J + %s_prim_count * 2], B_y = %s_prim_info[J + %s_prim_count * 3], B_z = %s_prim_info[J + %s_prim_count * 4];" % ((b,) * 6))
-    o.append("            zeta = ai + aj; inv_S1 = 1.0 / zeta;")
+    o.append("            zeta = ai + aj; inv_S1 = 1.0 / zeta; sq_zeta = sqrt(zeta);")
     o.append("            P_x = (ai * A_x + aj * B_x) * inv_S1; P_y = (ai * A_y + aj * B_y) * inv_S1; P_z = (ai * A_z + aj * B_z) * inv_S1;")
     o.append("            PA_x = (aj * inv_S1) * (B_x - A_x); PA_y = (aj * inv_S1) * (B_y - A_y); PA_z = (aj * inv_S1) * (B_z - A_z);")
     o.append("            AB_x = A_x - B_x; AB_y = A_y - B_y; AB_z = A_z - B_z;")
@@ -567,6 +597,8 @@ def emit_k(cls, mixed=False, batched=False):
         o.append("                cgJ[cb] = %s_prim_aoinds[(jj / NB) + %s_prim_count * (jj %% NB)] - aoff;" % (b, b))
     o.append("            }")
     o.append("        }")
+    if mixed:
+        o.append("        bool lo_mode = false;")
     o.append("        for (uint32_t t0 = 0; t0 < nL; t0 += KT) {")
     o.append("            if (!__any(active)) break;")
     o.append("            __syncthreads();")
@@ -579,7 +611,7 @@ def emit_k(cls, mixed=False, batched=False):
     o.append("                    const double al = %s_prim_info[Lp];" % d)
     o.append("                    const double D_x = %s_prim_info[Lp + %s_prim_count * 2], D_y = %s_prim_info[Lp + %s_prim_count * 3], D_z = %s_prim_info[Lp + %s_prim_count * 4];" % ((d,) * 6))
     o.append("                    const double et = ak + al, iet = 1.0 / et;")
-    o.append("                    R[1] = et; R[2] = iet;")
+    o.append("                    R[1] = et; R[2] = iet; R[12] = sqrt(et);")
     o.append("                    R[3] = (ak * C_x + al * D_x) * iet; R[4] = (ak * C_y + al * D_y) * iet; R[5] = (ak * C_z + al * D_z) * iet;")
     o.append("                    R[6] = (al * iet) * (D_x - C_x); R[7] = (al * iet) * (D_y - C_y); R[8] = (al * iet) * (D_z - C_z);")
     o.append("                    R[9] = C_x - D_x; R[10] = C_y - D_y; R[11] = C_z - D_z;")
@@ -591,8 +623,8 @@ def emit_k(cls, mixed=False, batched=False):
     o.append("                            const uint32_t pos = displ_k[cc] + lp * ND + cdd;")
     o.append("                            const double q = fabs(Q_K_%s[pos]);" % cd)
     o.append("                            qk = fmax(qk, q);")
-    o.append("                            R[12 + cc * ND + cdd] = pair_data_K_%s[pos];" % cd)
-    o.append("                            R[12 + NCD + cc * ND + cdd] = q;")
+    o.append("                            R[%d + cc * ND + cdd] = pair_data_K_%s[pos];" % (KREC0, cd))
+    o.append("                            R[%d + NCD + cc * ND + cdd] = q;" % KREC0)
     o.append("                        }")
     o.append("                    }")
     o.append("                    R[0] = qk;")
@@ -609,6 +641,9 @@ def emit_k(cls, mixed=False, batched=False):
     o.append("            __syncthreads();")
     o.append("            const uint32_t nt = (nL - t0 < (uint32_t)KT) ? (nL - t0) : (uint32_t)KT;")
     def kbody(tyq, ind, groups):
+        # the density gathers follow the Boys evaluation: vector loads complete
+        # in order, so gathers issued before the Boys table loads only lengthen
+        # the wait for the table and the live ranges
         lines = quartet_prologue(L, ty_geo=tyq, omega=True, ind=ind)
         TAU = (lambda t: "tau[%d * TILE_SIZE_K + lid]" % t) if heavy else (lambda t: "tau[%d]" % t)
         SAB = (lambda t: "Sab[%d * TILE_SIZE_K + lid]" % t) if heavy else (lambda t: "Sab[%d]" % t)
@@ -627,7 +662,7 @@ def emit_k(cls, mixed=False, batched=False):
         for icc in sorted(set(c_ for g_ in groups for c_ in g_)):
             for idd in range(nd):
                 tcd = icc * nd + idd
-                lines.append(ind + "const double Sc%d = R[%d], Qk%d = R[%d];" % (tcd, 12 + tcd, tcd, 12 + ncd + tcd))
+                lines.append(ind + "const double Sc%d = R[%d], Qk%d = R[%d];" % (tcd, KREC0 + tcd, tcd, KREC0 + ncd + tcd))
         for isl, group in enumerate(groups):
             if isl:
                 lines += [ind + x for x in launder(L)]
@@ -648,46 +683,33 @@ def emit_k(cls, mixed=False, batched=False):
             lines += [ind + x for x in _dag_lines(dag)]
         return lines
 
-    def sloop(groups, act):
-        """One loop over the staged ket records evaluating the passes ``groups``."""
-        q = []
-        q.append("            for (uint32_t s = 0; s < nt; ++s) {")
-        q.append("                const double* R = ket + s * REC;")
-        q.append("                if (%s && qij * R[0] * maxD * (1.0 + 1e-13) <= thr) %s = false;" % (act, act))
-        q.append("                if (!__any(%s)) break;" % act)
-        q.append("                if (%s) {" % act)
-        q.append("                    const double eta = R[1], inv_eta = R[2];")
-        q.append("                    const double Q_x = R[3], Q_y = R[4], Q_z = R[5];")
-        q.append("                    const double QC_x = R[6], QC_y = R[7], QC_z = R[8];")
-        q.append("                    const double CD_x = R[9], CD_y = R[10], CD_z = R[11];")
-        if mixed:
-            q.append("                    const bool lo = qij * R[0] * maxD < fp32_threshold;")
-            q.append("                    if (__all(lo)) {")
-            q.extend(kbody("float", "                        ", groups))
-            q.append("                    } else {")
-            q.extend(kbody("double", "                        ", groups))
-            q.append("                    }")
-        else:
-            q.extend(kbody("double", "                    ", groups))
-        q.append("                }")
-        q.append("            }")
-        return q
-
+    kscreen = "qij * R[0] * maxD * (1.0 + 1e-13) <= thr"
+    kbound = "qij * R[0] * maxD"
     if heavy:
         # heavy classes: one loop over the tile per pass of ket-first-index
         # components, so that only one pass's recursion is live at a time; the
         # screening sequence is the same in every pass, so each pass stops at
         # the same record
         o.append("            bool act_end = active;")
+        if mixed:
+            o.append("            bool lo_end = lo_mode;")
         for isl, group in enumerate(kslices):
             o.append("            {")
             o.append("            bool act%d = active;" % isl)
-            o += sloop([group], "act%d" % isl)
+            if mixed:
+                o.append("            bool lo%d = lo_mode;" % isl)
+            o += ket_loops("            ", "act%d" % isl, kscreen, kbound, KET_HEADER,
+                           lambda ty, ind, g=group: kbody(ty, ind, [g]), mixed, "lo%d" % isl)
             o.append("            act_end = act%d;" % isl)
+            if mixed:
+                o.append("            lo_end = lo%d;" % isl)
             o.append("            }")
         o.append("            active = act_end;")
+        if mixed:
+            o.append("            lo_mode = lo_end;")
     else:
-        o += sloop([list(range(nc))], "active")
+        o += ket_loops("            ", "active", kscreen, kbound, KET_HEADER,
+                       lambda ty, ind: kbody(ty, ind, [list(range(nc))]), mixed, "lo_mode")
     o.append("        }")
     o.append("    }")
     o.append("    #pragma unroll")
@@ -748,11 +770,11 @@ def emit_schwarz(pc):
     o.append("    const double ai = a_info[I], aj = b_info[J];")
     o.append("    const double A_x = a_info[I + a_count * 2], A_y = a_info[I + a_count * 3], A_z = a_info[I + a_count * 4];")
     o.append("    const double B_x = b_info[J + b_count * 2], B_y = b_info[J + b_count * 3], B_z = b_info[J + b_count * 4];")
-    o.append("    const double zeta = ai + aj, inv_S1 = 1.0 / zeta;")
+    o.append("    const double zeta = ai + aj, inv_S1 = 1.0 / zeta, sq_zeta = sqrt(zeta);")
     o.append("    const double P_x = (ai * A_x + aj * B_x) * inv_S1, P_y = (ai * A_y + aj * B_y) * inv_S1, P_z = (ai * A_z + aj * B_z) * inv_S1;")
     o.append("    const double PA_x = (aj * inv_S1) * (B_x - A_x), PA_y = (aj * inv_S1) * (B_y - A_y), PA_z = (aj * inv_S1) * (B_z - A_z);")
     o.append("    const double AB_x = A_x - B_x, AB_y = A_y - B_y, AB_z = A_z - B_z;")
-    o.append("    const double eta = zeta, inv_eta = inv_S1;")
+    o.append("    const double eta = zeta, inv_eta = inv_S1, sq_eta = sq_zeta;")
     o.append("    const double Q_x = P_x, Q_y = P_y, Q_z = P_z;")
     o.append("    const double QC_x = PA_x, QC_y = PA_y, QC_z = PA_z;")
     o.append("    const double CD_x = AB_x, CD_y = AB_y, CD_z = AB_z;")

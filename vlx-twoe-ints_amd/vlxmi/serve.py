@@ -127,10 +127,12 @@ class FockServer:
         return self.batch_runner(entries, [r.D for r in reqs])
 
     # ------------------------------------------------------------ loop
-    def serve(self, requests, realtime=True):
+    def serve(self, requests, realtime=True, progress=None, progress_every=100):
         """Serve ``requests`` (sorted by arrival).  With ``realtime`` the loop
         follows the wall clock (requests become visible at their arrival time);
-        otherwise every request is visible at once (offline batching)."""
+        otherwise every request is visible at once (offline batching).
+        ``progress(msg)`` is called each time another ``progress_every``
+        requests have completed."""
         reqs = sorted(requests, key=lambda r: r.arrival)
         pending = collections.deque()
         nxt = 0
@@ -172,6 +174,8 @@ class FockServer:
                 r.t_done = t_done
                 if self.keep_results:
                     r.J, r.K = (J.clone(), K.clone()) if hasattr(J, "clone") else (J, K)
+            if progress is not None and (done + len(batch)) // progress_every > done // progress_every:
+                progress("served %d of %d requests" % (done + len(batch), len(reqs)))
             done += len(batch)
             batches.append(len(batch))
             self.batch_log.append([r.rid for r in batch])

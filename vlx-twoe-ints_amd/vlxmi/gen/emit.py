@@ -34,6 +34,9 @@ LV = {"s": 0, "p": 1, "d": 2}
 NC = {"s": 1, "p": 3, "d": 6}
 
 KET_TILE = 64
+# LDS a single-wavefront workgroup may use without lowering the occupancy of a
+# kernel that needs all 512 registers (160 KB per CU / 4 SIMDs)
+LDS_PER_WAVE_FULL_REGS = 40960
 # ket record (LDS, doubles): 0 screening bound, 1 eta, 2 1/eta, 3-5 Q, 6-8 QC,
 # 9-11 CD, 12 sqrt(eta), then the per-component values from KREC0
 KREC0 = 13
@@ -292,7 +295,16 @@ def _strip_restrict(sig):
 # J kernel
 # --------------------------------------------------------------------------
 
-def emit_j(cls, mixed=False, batched=False):
+def ket_record_bytes(kind, cls):
+    """LDS bytes per staged ket record (the ket-tile size unit)."""
+    c, d = cls[2], cls[3]
+    ncd = NC[c] * NC[d]
+    rec = KREC0 + 2 * ncd
+    rec += rec % 2
+    return rec * 8 + (NC[d] * 4 if kind == "K" else 0)
+
+
+def emit_j(cls, mixed=False, batched=False, kt=None):
     """Source of computeCoulombFock{CLS} (or its _mixed / _batched variants) and its
     C-ABI launcher.  The batched variant is persistent: workgroup w processes
     work items w, w + gridDim.x, ...; item t = (bra block begin, bra block end
@@ -315,7 +327,7 @@ def emit_j(cls, mixed=False, batched=False):
     o.append("%s(%s)" % (name, ",\n    ".join(sig)))
     o.append("{")
     o.append("    constexpr int NA = %d, NB = %d, NC = %d, ND = %d, NAB = %d, NCD = %d;" % (na, nb, nc, nd, nab, ncd))
-    o.append("    constexpr int KT = %d, REC = %d;" % (KET_TILE, rec))
+    o.append("    constexpr int KT = %d, REC = %d;" % (kt or KET_TILE, rec))
     o.append("    __shared__ double ket[KT * REC];")
     o.append("    const uint32_t lane = threadIdx.x;")
     o.append("    const uint32_t nbra = %s_prim_pair_count_local / NAB;" % ab)
@@ -475,7 +487,7 @@ def emit_j(cls, mixed=False, batched=False):
 # K kernel
 # --------------------------------------------------------------------------
 
-def emit_k(cls, mixed=False, batched=False):
+def emit_k(cls, mixed=False, batched=False, kt=None, layout="reg"):
     """Source of computeExchangeFock{CLS} (or its _mixed / _batched variants) and
     its launcher.  The batched variant is persistent over a list of (i,k) blocks
     of lists that concatenate many molecules; per block, blk_mol gives the
@@ -503,12 +515,27 @@ def emit_k(cls, mixed=False, batched=False):
     o.append("%s(%s)" % (name, ",\n    ".join(sig)))
     o.append("{")
     o.append("    constexpr int NA = %d, NB = %d, NC = %d, ND = %d, NAC = %d, NAB = %d, NCD = %d;" % (na, nb, nc, nd, nac, nab, ncd))
-    o.append("    constexpr int KT = %d, REC = %d;" % (32 if heavy else KET_TILE, rec))
+    o.append("    constexpr int KT = %d, REC = %d;" % (kt or (32 if heavy else KET_TILE), rec))
     o.append("    __shared__ double ket[KT * REC];")
     o.append("    __shared__ uint32_t ket_cg[KT * ND];")
+    # heavy classes keep K accumulators in LDS when they fit beside the
+    # thresholds, S_ab and an 8-record ket tile within one wave's LDS share
+    # Heavy classes have two layouts, chosen per class by measurement:
+    #   "reg": K accumulators in registers, a-major digestion, the pass's density
+    #          gathers loaded together after the Boys evaluation;
+    #   "lds": K accumulators in LDS (a pass keeps its own NA in registers) when
+    #          they fit beside the thresholds, S_ab and an 8-record ket tile in one
+    #          wave's LDS share, and b-major digestion with each b's density row
+    #          loaded just before its block.
+    lds_layout = heavy and layout == "lds"
+    lacc = lds_layout and (16 * nab + 8 * nac) * 64 + 8 * ket_record_bytes("K", cls) <= LDS_PER_WAVE_FULL_REGS
     if heavy:
         # per-lane thresholds and S_ab live in LDS ([component][lane]) to free registers
         o.append("    __shared__ double ltau[NAB * TILE_SIZE_K], lSab[NAB * TILE_SIZE_K];")
+    if lacc:
+        # per-lane K accumulators in LDS ([component][lane]); a pass keeps its
+        # own NA accumulators in registers for the duration of its ket loop
+        o.append("    __shared__ double lacc[NAC * TILE_SIZE_K];")
     o.append("    const uint32_t lane = threadIdx.x;")
     if batched:
         o.append("    (void)%s_max_D;" % bd)
@@ -543,9 +570,13 @@ def emit_k(cls, mixed=False, batched=False):
     o.append("    for (int t = 0; t < NC; ++t) displ_k[t] = pair_displs_K_%s[K * NC + t];" % cd)
     o.append("    const uint32_t nJ = pair_counts_K_%s[I * NA] / NB;" % ab)
     o.append("    const uint32_t nL = pair_counts_K_%s[K * NC] / ND;" % cd)
-    o.append("    double acc[NAC];")
-    o.append("    #pragma unroll")
-    o.append("    for (int t = 0; t < NAC; ++t) acc[t] = 0.0;")
+    if lacc:
+        o.append("    #pragma unroll")
+        o.append("    for (int t = 0; t < NAC; ++t) lacc[t * TILE_SIZE_K + lane] = 0.0;")
+    else:
+        o.append("    double acc[NAC];")
+        o.append("    #pragma unroll")
+        o.append("    for (int t = 0; t < NAC; ++t) acc[t] = 0.0;")
     o.append("    for (uint32_t jc = 0; jc < nJ; jc += TILE_SIZE_K) {")
     o.append("        const uint32_t jp = jc + lane;")
     o.append("        bool active = jp < nJ;")
@@ -654,10 +685,15 @@ def emit_k(cls, mixed=False, batched=False):
             lines += [ind + "uint32_t lid = lane;", ind + "asm volatile(\"\" : \"+v\"(lid));"]
             for ibb in range(nb):
                 lines += [ind + "uint32_t rowJ%d = cgJ[%d];" % (ibb, ibb), ind + "asm volatile(\"\" : \"+v\"(rowJ%d));" % ibb]
-        for ibb in range(nb):
-            for idd in range(nd):
-                lines.append(ind + "const double Dj%d_%d = Dm[(size_t)%s * nao + ket_cg[s * ND + %d]];"
-                             % (ibb, idd, ("rowJ%d" % ibb) if heavy else ("cgJ[%d]" % ibb), idd))
+        # b-major digestion: each b's density row loaded just before its block
+        # (an opaque row index keeps the loads from being hoisted)
+        bouter = lds_layout
+        ACC = (lambda t: "pacc%d" % t) if lacc else (lambda t: "acc[%d]" % t)
+        if not bouter:
+            for ibb in range(nb):
+                for idd in range(nd):
+                    lines.append(ind + "const double Dj%d_%d = Dm[(size_t)%s * nao + ket_cg[s * ND + %d]];"
+                                 % (ibb, idd, ("rowJ%d" % ibb) if heavy else ("cgJ[%d]" % ibb), idd))
         # digestion: acc[a,c] += Sab * sum_{b,d} mask * X * Sc * D  (FP64 accumulation)
         for icc in sorted(set(c_ for g_ in groups for c_ in g_)):
             for idd in range(nd):
@@ -669,17 +705,27 @@ def emit_k(cls, mixed=False, batched=False):
             dag = QuartetDAG(la, lb, lc, ld, ty=tyq, rule_vrr="first", rule_hrr="first", prefix="q%d_" % isl)
             for ic in group:
                 cc = tri(lc)[ic]
-                for ia, aa in enumerate(tri(la)):
+                order = ([(ia, ib) for ib in range(nb) for ia in range(na)] if bouter
+                         else [(ia, ib) for ia in range(na) for ib in range(nb)])
+                for ia, ib in order:
+                    aa, bbx = tri(la)[ia], tri(lb)[ib]
                     tac = ia * nc + ic
-                    for ib, bbx in enumerate(tri(lb)):
-                        tab = ia * nb + ib
-                        terms = []
-                        for idd, dd in enumerate(tri(ld)):
-                            tcd = ic * nd + idd
-                            x = dag.X(aa, bbx, cc, dd)
-                            xv = x if tyq == "double" else "double(%s)" % x
-                            terms.append("((Qk%d > %s) ? (%s * Sc%d * Dj%d_%d) : 0.0)" % (tcd, TAU(tab), xv, tcd, ib, idd))
-                        dag.lines.append("acc[%d] += %s * (%s);" % (tac, SAB(tab), " + ".join(terms)))
+                    tab = ia * nb + ib
+                    dname = (lambda b_, d_: "Dj%d_%d_c%d" % (b_, d_, ic)) if bouter else (lambda b_, d_: "Dj%d_%d" % (b_, d_))
+                    if bouter and ia == 0:
+                        # this b's density row, loaded just before its block
+                        dag.lines.append("uint32_t rowB%d_%d_c%d = cgJ[%d];" % (isl, ib, ic, ib))
+                        dag.lines.append("asm volatile(\"\" : \"+v\"(rowB%d_%d_c%d));" % (isl, ib, ic))
+                        for idd in range(nd):
+                            dag.lines.append("const double %s = Dm[(size_t)rowB%d_%d_c%d * nao + ket_cg[s * ND + %d]];"
+                                             % (dname(ib, idd), isl, ib, ic, idd))
+                    terms = []
+                    for idd, dd in enumerate(tri(ld)):
+                        tcd = ic * nd + idd
+                        x = dag.X(aa, bbx, cc, dd)
+                        xv = x if tyq == "double" else "double(%s)" % x
+                        terms.append("((Qk%d > %s) ? (%s * Sc%d * %s) : 0.0)" % (tcd, TAU(tab), xv, tcd, dname(ib, idd)))
+                    dag.lines.append("%s += %s * (%s);" % (ACC(tac), SAB(tab), " + ".join(terms)))
             lines += [ind + x for x in _dag_lines(dag)]
         return lines
 
@@ -698,8 +744,18 @@ def emit_k(cls, mixed=False, batched=False):
             o.append("            bool act%d = active;" % isl)
             if mixed:
                 o.append("            bool lo%d = lo_mode;" % isl)
+            if lacc:
+                for ic in group:
+                    for ia in range(na):
+                        t = ia * nc + ic
+                        o.append("            double pacc%d = lacc[%d * TILE_SIZE_K + lane];" % (t, t))
             o += ket_loops("            ", "act%d" % isl, kscreen, kbound, KET_HEADER,
                            lambda ty, ind, g=group: kbody(ty, ind, [g]), mixed, "lo%d" % isl)
+            if lacc:
+                for ic in group:
+                    for ia in range(na):
+                        t = ia * nc + ic
+                        o.append("            lacc[%d * TILE_SIZE_K + lane] = pacc%d;" % (t, t))
             o.append("            act_end = act%d;" % isl)
             if mixed:
                 o.append("            lo_end = lo%d;" % isl)
@@ -714,7 +770,7 @@ def emit_k(cls, mixed=False, batched=False):
     o.append("    }")
     o.append("    #pragma unroll")
     o.append("    for (int t = 0; t < NAC; ++t) {")
-    o.append("        double v = acc[t];")
+    o.append("        double v = %s;" % ("lacc[t * TILE_SIZE_K + lane]" if lacc else "acc[t]"))
     o.append("        #pragma unroll")
     o.append("        for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);")
     o.append("        if (lane == 0) mat_K[e0 + t] += v;")

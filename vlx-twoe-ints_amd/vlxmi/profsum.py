@@ -1,9 +1,14 @@
-"""Summarise a rocprofv3 ``--kernel-trace --stats`` result as a Markdown table.
+"""Summarise rocprofv3 results as Markdown tables.
 
-Reads either the SQLite database rocprofv3 writes by default
+Kernel trace: reads either the SQLite database rocprofv3 writes by default
 (``*_results.db``, view ``top_kernels``) or a ``*kernel_stats.csv`` file.
 
     python -m vlxmi.profsum RESULT [--command "the profiled command"] [--top N]
+
+Counters: reads the ``mem``, ``fetch`` and ``write`` passes (``--output-format
+csv``) of one command under PMC_DIR and the per-class JSON bench.py writes.
+
+    python -m vlxmi.profsum --counters PMC_DIR --per-class PERCLASS.json [--top N]
 """
 
 from __future__ import annotations
@@ -50,19 +55,6 @@ def markdown(path: str, command: str = "", top: int = 30) -> str:
     return "\n".join(o) + "\n"
 
 
-def main():
-    ap = argparse.ArgumentParser()
-    ap.add_argument("result")
-    ap.add_argument("--command", default="")
-    ap.add_argument("--top", type=int, default=30)
-    a = ap.parse_args()
-    print(markdown(a.result, a.command, a.top), end="")
-
-
-if __name__ == "__main__":
-    main()
-
-
 # --------------------------------------------------------------------------
 # counter collections (rocprofv3 --pmc ... --output-format csv)
 # --------------------------------------------------------------------------
@@ -102,3 +94,65 @@ def copy_calibration(fetch_path, write_path, copy_bytes):
 
     f, w = pick(fe, "FETCH_SIZE"), pick(wr, "WRITE_SIZE")
     return (copy_bytes / f if f else None), (copy_bytes / w if w else None)
+
+
+def counters_markdown(pmc_dir, per_class_path, top=25, copy_bytes=512 << 20):
+    """Per-kernel memory-pipeline and HBM summary of the counter passes.
+
+    TA/TD busy: TA_TA_BUSY / TD_TD_BUSY summed over the per-CU instances,
+    divided by (GRBM_GUI_ACTIVE / 8 XCDs) x 256 CUs -- the fraction of the
+    kernel's cycles the address / data units of an average CU were busy.
+    HBM bytes: FETCH_SIZE and WRITE_SIZE scaled by what they read for the copy
+    kernel of ``copy_bytes`` bytes in the same runs."""
+    import json
+    import os
+    mem = counters_from(os.path.join(pmc_dir, "mem", "run_counter_collection.csv"))
+    fe_p = os.path.join(pmc_dir, "fetch", "run_counter_collection.csv")
+    wr_p = os.path.join(pmc_dir, "write", "run_counter_collection.csv")
+    kf, kw = copy_calibration(fe_p, wr_p, copy_bytes)
+    fe, wr = counters_from(fe_p), counters_from(wr_p)
+    with open(per_class_path) as f:
+        pc = json.load(f)
+
+    def by_name(d):
+        out = {}
+        for (_, name), v in d.items():
+            out.setdefault(name, v)
+        return out
+
+    mem, fe, wr = by_name(mem), by_name(fe), by_name(wr)
+    names = sorted((n for n in mem if n in pc and not n.endswith("_mixed")), key=lambda n: -pc[n]["avg_s"])[:top]
+    o = ["Copy-kernel calibration: %.1f bytes per FETCH_SIZE unit, %.1f bytes per WRITE_SIZE unit." % (kf, kw), "",
+         "| kernel | ms | TFLOP/s (alg.) | TA busy | TD busy | VALU instr / quartet | VMEM rd / quartet "
+         "| HBM read GB | HBM write GB | HBM GB/s |",
+         "|---|---:|---:|---:|---:|---:|---:|---:|---:|---:|"]
+    for n in names:
+        c = mem[n]
+        g = c.get("GRBM_GUI_ACTIVE", 0.0) / 8.0 * 256.0
+        q = pc[n]["quartets"] / 64.0 or 1.0
+        rb = fe.get(n, {}).get("FETCH_SIZE", 0.0) * kf
+        wb = wr.get(n, {}).get("WRITE_SIZE", 0.0) * kw
+        t = pc[n]["avg_s"]
+        o.append("| %s | %.1f | %.2f | %.2f | %.2f | %.1f | %.2f | %.2f | %.2f | %.0f |" % (
+            n, 1e3 * t, pc[n]["tflops"], c.get("TA_TA_BUSY", 0.0) / g if g else 0.0,
+            c.get("TD_TD_BUSY", 0.0) / g if g else 0.0, c.get("SQ_INSTS_VALU", 0.0) / q,
+            c.get("SQ_INSTS_VMEM_RD", 0.0) / q, rb / 1e9, wb / 1e9, (rb + wb) / t / 1e9))
+    return "\n".join(o) + "\n"
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("result", nargs="?")
+    ap.add_argument("--command", default="")
+    ap.add_argument("--top", type=int, default=30)
+    ap.add_argument("--counters", default=None, help="directory holding the mem/fetch/write counter passes")
+    ap.add_argument("--per-class", default=None, help="per-class JSON written by bench.py --per-class-out")
+    a = ap.parse_args()
+    if a.counters:
+        print(counters_markdown(a.counters, a.per_class, a.top), end="")
+    else:
+        print(markdown(a.result, a.command, a.top), end="")
+
+
+if __name__ == "__main__":
+    main()

@@ -304,7 +304,7 @@ def ket_record_bytes(kind, cls):
     return rec * 8 + (NC[d] * 4 if kind == "K" else 0)
 
 
-def emit_j(cls, mixed=False, batched=False, kt=None):
+def emit_j(cls, mixed=False, batched=False, kt=None, layout="reg"):
     """Source of computeCoulombFock{CLS} (or its _mixed / _batched variants) and its
     C-ABI launcher.  The batched variant is persistent: workgroup w processes
     work items w, w + gridDim.x, ...; item t = (bra block begin, bra block end
@@ -350,10 +350,23 @@ def emit_j(cls, mixed=False, batched=False, kt=None):
     o.append("    double zeta = 1.0, sq_zeta = 1.0, inv_S1 = 1.0, P_x = 0.0, P_y = 0.0, P_z = 0.0;")
     o.append("    double PA_x = 0.0, PA_y = 0.0, PA_z = 0.0, AB_x = 0.0, AB_y = 0.0, AB_z = 0.0;")
     o.append("    double qmax = 0.0;")
-    o.append("    double tau[NAB];")
-    o.append("    double acc[NAB];")
-    o.append("    #pragma unroll")
-    o.append("    for (int t = 0; t < NAB; ++t) { acc[t] = 0.0; tau[t] = 0.0; }")
+    jsl = plan_slices(la, lb, lc, ld, "J")
+    # Classes split into passes have two layouts, chosen per class by
+    # measurement: "reg" keeps the NAB accumulators and thresholds in registers;
+    # "lds" keeps them in LDS ([component][lane], lane-private slots), the
+    # digestion adding each term with one LDS f64 add and reading the
+    # threshold per use, which frees 4*NAB registers for the recursion.
+    jlds = len(jsl) > 1 and layout == "lds"
+    if jlds:
+        o.append("    __shared__ double ltau[NAB * TILE_SIZE_J], lacc[NAB * TILE_SIZE_J];")
+        o.append("    double* tau = ltau;")
+        o.append("    #pragma unroll")
+        o.append("    for (int t = 0; t < NAB; ++t) { lacc[t * TILE_SIZE_J + lane] = 0.0; tau[t * TILE_SIZE_J + lane] = 0.0; }")
+    else:
+        o.append("    double tau[NAB];")
+        o.append("    double acc[NAB];")
+        o.append("    #pragma unroll")
+        o.append("    for (int t = 0; t < NAB; ++t) { acc[t] = 0.0; tau[t] = 0.0; }")
     o.append("    if (active) {")
     o.append("        const uint32_t e0 = bb * NAB;")
     o.append("        const uint32_t I = %s_first_inds_local[e0] / NA;" % ab)
@@ -370,7 +383,7 @@ def emit_j(cls, mixed=False, batched=False, kt=None):
     o.append("        for (int t = 0; t < NAB; ++t) {")
     o.append("            const double q = fabs(%s_mat_Q_local[e0 + t]);" % ab)
     o.append("            qmax = fmax(qmax, q);")
-    o.append("            tau[t] = thr / q;")
+    o.append("            tau[%s] = thr / q;" % ("t * TILE_SIZE_J + lane" if jlds else "t"))
     o.append("        }")
     o.append("    }")
     if mixed:
@@ -415,6 +428,8 @@ def emit_j(cls, mixed=False, batched=False, kt=None):
 
     def jbody(tyq, ind, groups):
         lines = quartet_prologue(L, ty_geo=tyq, ind=ind)
+        if jlds:
+            lines += [ind + "uint32_t lid = lane;", ind + "asm volatile(\"\" : \"+v\"(lid));"]
         for isl, group in enumerate(groups):
             if isl:
                 lines += [ind + x for x in launder(L)]
@@ -430,14 +445,17 @@ def emit_j(cls, mixed=False, batched=False, kt=None):
                             tab = ia * nb + ib
                             x = dag.X(aa, bbb, cc, dd)
                             xv = x if tyq == "double" else "double(%s)" % x
-                            dag.lines.append("acc[%d] += (Qd%d > tau[%d]) ? (%s * Dp%d) : 0.0;" % (tab, tcd, tab, xv, tcd))
+                            if jlds:
+                                dag.lines.append("if (Qd%d > ltau[%d * TILE_SIZE_J + lid]) __hip_atomic_fetch_add(&lacc[%d * TILE_SIZE_J + lid], %s * Dp%d, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);"
+                                                 % (tcd, tab, tab, xv, tcd))
+                            else:
+                                dag.lines.append("acc[%d] += (Qd%d > tau[%d]) ? (%s * Dp%d) : 0.0;" % (tab, tcd, tab, xv, tcd))
             lines += [ind + x for x in dig]
             lines += [ind + x for x in _dag_lines(dag)]
         return lines
 
     # J keeps its passes inside one ket loop (the bra-side state is small);
     # measured faster than one loop per pass
-    jsl = plan_slices(la, lb, lc, ld, "J")
     o += ket_loops("        ", "active", "qmax * R[0] * (1.0 + 1e-13) <= thr", "qmax * R[0]", KET_HEADER,
                    lambda ty, ind: jbody(ty, ind, jsl), mixed, "lo_mode")
     o.append("    }")
@@ -445,7 +463,7 @@ def emit_j(cls, mixed=False, batched=False, kt=None):
     o.append("        const uint32_t e0 = bb * NAB;")
     o.append("        #pragma unroll")
     o.append("        for (int t = 0; t < NAB; ++t) {")
-    o.append("            const double v = %s_pair_data_local[e0 + t] * acc[t];" % ab)
+    o.append("            const double v = %s_pair_data_local[e0 + t] * %s;" % (ab, "lacc[t * TILE_SIZE_J + lane]" if jlds else "acc[t]"))
     if batched:
         o.append("            mat_J[e0 + t] += v;")
     else:

@@ -41,10 +41,12 @@ def test_header_is_current():
 def test_boys_table_fill_matches_oracle():
     table, ft = runtime.boys_tables_host()
     t = table.reshape(-1, 16)
-    for k in (0, 1, 37, 400, 799, 800):
-        F = oracle.boys(14, 0.05 * k)
+    n = t.shape[0]
+    dt = 40.0 / (n - 1)            # the grid covers T in [0, 40]
+    for k in (0, 1, 37, n // 2, n - 2, n - 1):
+        F = oracle.boys(14, dt * k)
         assert np.max(np.abs(t[k, :15] - F) / F) < 2e-15
-        assert abs(t[k, 15] - np.exp(-0.05 * k)) <= 1e-16 * max(1.0, np.exp(-0.05 * k))
+        assert abs(t[k, 15] - np.exp(-dt * k)) <= 1e-16 * max(1.0, np.exp(-dt * k))
     assert np.allclose(ft, 1.0 / (2 * np.arange(32) + 1))
 
 

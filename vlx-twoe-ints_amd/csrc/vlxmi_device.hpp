@@ -11,9 +11,17 @@
 //   boys_func_ft: 1/(2m+1), m = 0..31 (the generated kernels fold these
 //     constants at compile time; the array is kept for the signature).
 //
-// For T < VLX_BOYS_TMAX: a 7-term Taylor expansion of F_L about the nearest
-// grid point (|T - T_k| <= DT/2), exp(-T) = exp(-T_k) * (8-term Taylor of
-// exp(-d)), then downward recursion F_{m} = (2T F_{m+1} + exp(-T)) / (2m+1).
+// For T < VLX_BOYS_TMAX: a 6-term Taylor expansion of F_L about the nearest
+// grid point, |T - T_k| <= DT/2 = 0.01, so the remainder is below
+// F_L * 0.01^6 / 6! = 1.4e-15 F_L (F_{m+1} <= F_m); the six coefficients
+// F_L..F_{L+5}(T_k) are contiguous, three 16-byte loads per lane -- the table
+// gather is what limits the light classes, so the row is kept that short.
+// exp(-T): for L <= 4 from the device exp (those classes are bound by the
+// table gathers, not by instructions); for L >= 5, whose kernels re-evaluate
+// the prologue once per component pass and are not gather-bound, as
+// exp(-T_k) (column 15) times a 7-term Taylor of exp(T_k - T) (remainder
+// < 0.01^7/7! = 2e-18).  Then downward recursion
+// F_{m} = (2T F_{m+1} + exp(-T)) / (2m+1).
 // For T >= VLX_BOYS_TMAX: F_0 = sqrt(pi/T)/2 (Newton-refined 1/sqrt) and upward recursion
 // F_{m+1} = ((2m+1) F_m - exp(-T)) / (2T); there exp(-T) < 5e-18 and the
 // single-precision hardware exponential is accurate enough for it.
@@ -25,9 +33,9 @@
 #include "vlxmi_status.h"
 
 #define VLX_BOYS_NCOL 16
-#define VLX_BOYS_NROW 801
-#define VLX_BOYS_DT 0.05
-#define VLX_BOYS_INV_DT 20.0
+#define VLX_BOYS_NROW 2001
+#define VLX_BOYS_DT 0.02
+#define VLX_BOYS_INV_DT 50.0
 #define VLX_BOYS_TMAX 40.0
 
 #define MATH_CONST_PI 3.14159265358979323846
@@ -69,9 +77,8 @@ __device__ __forceinline__ void boys(double T, const double* __restrict__ table,
         const int k = (int)(T * VLX_BOYS_INV_DT + 0.5);
         const double nd = (double)k * VLX_BOYS_DT - T;   // -(T - T_k)
         const double* row = table + k * VLX_BOYS_NCOL;
-        // F_L(T) = sum_j F_{L+j}(T_k) (-d)^j / j!, j = 0..6 (Horner)
-        double s = row[L + 6];
-        s = fma(s, nd * (1.0 / 6.0), row[L + 5]);
+        // F_L(T) = sum_j F_{L+j}(T_k) (-d)^j / j!, j = 0..5 (Horner)
+        double s = row[L + 5];
         s = fma(s, nd * (1.0 / 5.0), row[L + 4]);
         s = fma(s, nd * (1.0 / 4.0), row[L + 3]);
         s = fma(s, nd * (1.0 / 3.0), row[L + 2]);
@@ -79,16 +86,19 @@ __device__ __forceinline__ void boys(double T, const double* __restrict__ table,
         s = fma(s, nd, row[L]);
         F[L] = s;
         if (L > 0) {
-            // exp(-T) = exp(-T_k) * exp(nd), exp(nd) by 8 Taylor terms
-            double e = 1.0 / 5040.0;
-            e = fma(e, nd, 1.0 / 720.0);
-            e = fma(e, nd, 1.0 / 120.0);
-            e = fma(e, nd, 1.0 / 24.0);
-            e = fma(e, nd, 1.0 / 6.0);
-            e = fma(e, nd, 0.5);
-            e = fma(e, nd, 1.0);
-            e = fma(e, nd, 1.0);
-            const double et = row[15] * e;
+            double et;
+            if constexpr (L >= 5) {
+                double e = 1.0 / 720.0;
+                e = fma(e, nd, 1.0 / 120.0);
+                e = fma(e, nd, 1.0 / 24.0);
+                e = fma(e, nd, 1.0 / 6.0);
+                e = fma(e, nd, 0.5);
+                e = fma(e, nd, 1.0);
+                e = fma(e, nd, 1.0);
+                et = row[15] * e;
+            } else {
+                et = exp(-T);
+            }
             const double T2 = 2.0 * T;
 #pragma unroll
             for (int m = L - 1; m >= 0; --m) F[m] = fma(T2, F[m + 1], et) * (1.0 / (2.0 * m + 1.0));

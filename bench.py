@@ -265,6 +265,9 @@ def main():
     torch.cuda.synchronize()
     setup_s = time.time() - t0
     _progress("fock: %s lists and device data ready (%.1f s)" % (a.workload, setup_s))
+    if a.workload != "lattice64":
+        # builds of minutes: report every finished class launch
+        eng.progress = _progress
     Dd = eng.dev.tensor(D)
 
     def timed(precision):

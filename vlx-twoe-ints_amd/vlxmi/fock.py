@@ -42,6 +42,7 @@ class FockEngine:
         self.omega = float(omega)
         self.target_waves = target_waves
         self.timer = None          # list of (kernel name, start event, end event) when set
+        self.progress = None       # progress(msg): called after each class launch has finished (synchronizes)
         # "fp64": plain path; "mixed": quartets whose Schwarz x |D| bound is below
         # fp32_threshold run the VRR/HRR in FP32 (Boys, prefactors, sums in FP64)
         assert precision in ("fp64", "mixed")
@@ -169,6 +170,9 @@ class FockEngine:
         return e
 
     def _ev_end(self, e0, name):
+        if self.progress is not None:
+            self.torch.cuda.synchronize(self.dev.device)
+            self.progress("%s done" % name)
         if e0 is None:
             return
         e1 = self.torch.cuda.Event(enable_timing=True)

@@ -1,0 +1,68 @@
+"""Host logic of the build: ket-tile rule and the per-class tables it reads."""
+
+import json
+import os
+import re
+
+import pytest
+
+from _util import PKG  # noqa: F401  (puts the package on sys.path)
+from vlxmi.gen import build as B
+from vlxmi.gen import emit
+from vlxmi.pairs import j_classes, k_classes
+
+
+def test_tile_rule_reaches_register_occupancy():
+    # 512 registers -> 1 wave/SIMD -> 4 waves/CU -> LDS budget 40 KB
+    assert B.tile_for(512, 41216, 32, 712) == 16          # 29.8 KB after halving once
+    assert B.tile_for(512, 59648, 32, 712) == 4           # fixed part 36.9 KB leaves room for 4 records
+    # 252 registers -> 2 waves/SIMD -> 8 waves/CU -> 20 KB
+    assert B.tile_for(252, 44032, 64, 688) == 16
+    # 68 registers -> 7 waves/SIMD -> 28 waves/CU -> 5851 B
+    assert B.tile_for(68, 8192, 64, 128) == 32
+    # already within budget: unchanged
+    assert B.tile_for(512, 30000, 32, 712) == 32
+    # never below the minimum tile
+    assert B.tile_for(512, 100000, 32, 712) == 4
+
+
+def _heavy(kind, cls):
+    la, lb, lc, ld = [emit.LV[x] for x in cls]
+    return len(emit.plan_slices(la, lb, lc, ld, "J" if kind == "J" else "Kheavy")) > 1
+
+
+@pytest.mark.parametrize("path,kind", [(B.J_LAYOUT_TABLE, "J"), (B.LAYOUT_TABLE, "K")])
+def test_layout_tables_name_split_classes(path, kind):
+    lay = json.load(open(path))
+    classes = [ab + cd for ab, cd in j_classes()] if kind == "J" else list(k_classes())
+    for cls, v in lay.items():
+        assert cls in classes and v in ("reg", "lds")
+        assert _heavy(kind, cls), cls          # only classes that run in passes have two layouts
+
+
+def test_tile_table_entries():
+    tiles = json.load(open(B.TILE_TABLE))
+    names = set(B.sources(tiles={}, layouts={}))
+    for fn, kt in tiles.items():
+        assert fn in names
+        assert kt in (4, 8, 16, 32), (fn, kt)
+
+
+def test_sources_use_table_values():
+    tiles = B.load_tiles()
+    srcs = B.sources()
+    for fn, kt in list(tiles.items())[:20]:
+        m = re.search(r"constexpr int KT = (\d+)", srcs[fn])
+        assert m and int(m.group(1)) == kt
+
+
+def test_mixed_kernels_have_one_body_per_loop():
+    """The FP64 loop switches to the FP32 loop once; no per-record branch
+    between two bodies remains."""
+    for cls in ("ssss", "pdpd", "ddsp"):
+        src = emit.emit_j(cls, mixed=True)
+        assert src.count("if (__all(!active || qmax * R[0] < fp32_threshold)) { lo_mode = true; break; }") == 1
+        assert "if (__all(lo))" not in src
+    src = emit.emit_k("pddd", mixed=True)
+    npass = len(emit.plan_slices(1, 2, 2, 2, "Kheavy"))
+    assert src.count("< fp32_threshold)) { lo") == npass

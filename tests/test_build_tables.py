@@ -66,3 +66,20 @@ def test_mixed_kernels_have_one_body_per_loop():
     src = emit.emit_k("pddd", mixed=True)
     npass = len(emit.plan_slices(1, 2, 2, 2, "Kheavy"))
     assert src.count("< fp32_threshold)) { lo") == npass
+
+
+def test_density_order_table():
+    """Classes in k_dorder.json read D[l][j] (one density row per wave); all
+    others read D[j][l] as the reference does."""
+    tab = B.load_layouts(B.DORDER_TABLE)
+    assert tab and set(tab.values()) == {"lj"}
+    kc = list(k_classes())
+    srcs = B.sources()
+    for cls in kc:
+        for pre in ("k", "km", "kb", "kbm"):
+            s = srcs["%s_%s.hip" % (pre, cls)]
+            lj = "Dm[(size_t)ket_cg[s * ND + 0] * nao + " in s
+            jl = re.search(r"Dm\[\(size_t\)\w+(\[\d+\])? \* nao \+ ket_cg\[s \* ND \+ 0\]\]", s) is not None
+            assert (lj, jl) == ((True, False) if cls in tab else (False, True)), (pre, cls)
+    for cls in tab:
+        assert cls in kc

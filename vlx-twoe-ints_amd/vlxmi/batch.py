@@ -290,6 +290,8 @@ class BatchEngine:
             Dd = D if t.is_tensor(D) else self.dev.tensor(np.asarray(D, dtype=np.float64))
             if not e.identity_T:
                 Dd = e.T.t() @ Dd @ e.T
+            # exactly symmetric: some exchange classes read D_jl as D[l][j]
+            Dd = 0.5 * (Dd + Dd.t())
             parts.append(Dd.reshape(-1))
         return t.cat(parts).contiguous()
 

@@ -145,7 +145,9 @@ class FockEngine:
         t = self.torch
         if not t.is_tensor(D_sph):
             D_sph = self.dev.tensor(np.asarray(D_sph, dtype=np.float64))
-        return (self.T.t() @ D_sph @ self.T).contiguous()
+        Dc = self.T.t() @ D_sph @ self.T
+        # exactly symmetric: some exchange classes read D_jl as D[l][j]
+        return (0.5 * (Dc + Dc.t())).contiguous()
 
     def prepare_kets(self, Dc):
         """mat_D per J list and the ket-ordered copies (device)."""

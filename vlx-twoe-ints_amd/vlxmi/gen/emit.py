@@ -505,13 +505,20 @@ def emit_j(cls, mixed=False, batched=False, kt=None, layout="reg"):
 # K kernel
 # --------------------------------------------------------------------------
 
-def emit_k(cls, mixed=False, batched=False, kt=None, layout="reg"):
+def emit_k(cls, mixed=False, batched=False, kt=None, layout="reg", dorder="jl"):
     """Source of computeExchangeFock{CLS} (or its _mixed / _batched variants) and
     its launcher.  The batched variant is persistent over a list of (i,k) blocks
     of lists that concatenate many molecules; per block, blk_mol gives the
     molecule, whose density lives at mat_D_full_AO + mol_D_off[m] (mol_naos[m]
     Cartesian AOs, AO indices offset by mol_ao_off[m]) and whose maxD_bd is
-    mol_maxD[m]."""
+    mol_maxD[m].
+
+    ``dorder`` is how the density element D_jl of the digestion is read:
+    "jl" reads D[j][l] as the reference does (gen_exchange.py:323-342), so the
+    64 lanes of a wave (64 partners j, one ket partner l) read 64 rows; "lj"
+    reads D[l][j], so the wave reads one row of D at 64 columns.  The two agree
+    for a symmetric density (every SCF density; the host symmetrises it
+    exactly), and the faster one per class is a measured choice."""
     a, b, c, d = cls
     la, lb, lc, ld = LV[a], LV[b], LV[c], LV[d]
     na, nb, nc, nd = NC[a], NC[b], NC[c], NC[d]
@@ -689,6 +696,14 @@ def emit_k(cls, mixed=False, batched=False, kt=None, layout="reg"):
     o.append("            }")
     o.append("            __syncthreads();")
     o.append("            const uint32_t nt = (nL - t0 < (uint32_t)KT) ? (nL - t0) : (uint32_t)KT;")
+    assert dorder in ("jl", "lj")
+
+    def dread(jrow, idd):
+        """Density element D_jl of partner AO ``jrow`` and the staged ket AO idd."""
+        if dorder == "jl":
+            return "Dm[(size_t)%s * nao + ket_cg[s * ND + %d]]" % (jrow, idd)
+        return "Dm[(size_t)ket_cg[s * ND + %d] * nao + %s]" % (idd, jrow)
+
     def kbody(tyq, ind, groups):
         # the density gathers follow the Boys evaluation: vector loads complete
         # in order, so gathers issued before the Boys table loads only lengthen
@@ -710,8 +725,8 @@ def emit_k(cls, mixed=False, batched=False, kt=None, layout="reg"):
         if not bouter:
             for ibb in range(nb):
                 for idd in range(nd):
-                    lines.append(ind + "const double Dj%d_%d = Dm[(size_t)%s * nao + ket_cg[s * ND + %d]];"
-                                 % (ibb, idd, ("rowJ%d" % ibb) if heavy else ("cgJ[%d]" % ibb), idd))
+                    lines.append(ind + "const double Dj%d_%d = %s;"
+                                 % (ibb, idd, dread(("rowJ%d" % ibb) if heavy else ("cgJ[%d]" % ibb), idd)))
         # digestion: acc[a,c] += Sab * sum_{b,d} mask * X * Sc * D  (FP64 accumulation)
         for icc in sorted(set(c_ for g_ in groups for c_ in g_)):
             for idd in range(nd):
@@ -735,8 +750,8 @@ def emit_k(cls, mixed=False, batched=False, kt=None, layout="reg"):
                         dag.lines.append("uint32_t rowB%d_%d_c%d = cgJ[%d];" % (isl, ib, ic, ib))
                         dag.lines.append("asm volatile(\"\" : \"+v\"(rowB%d_%d_c%d));" % (isl, ib, ic))
                         for idd in range(nd):
-                            dag.lines.append("const double %s = Dm[(size_t)rowB%d_%d_c%d * nao + ket_cg[s * ND + %d]];"
-                                             % (dname(ib, idd), isl, ib, ic, idd))
+                            dag.lines.append("const double %s = %s;"
+                                             % (dname(ib, idd), dread("rowB%d_%d_c%d" % (isl, ib, ic), idd)))
                     terms = []
                     for idd, dd in enumerate(tri(ld)):
                         tcd = ic * nd + idd

@@ -132,7 +132,7 @@ def bench_batch(a):
                                            "atoms_total": int(sum(m.natoms for m, _ in mols))},
            "setup_s": setup}
     for prec in ("fp64", "mixed"):
-        be = BatchEngine(dev, precision=prec, fp32_threshold=a.fp32_threshold)
+        be = BatchEngine(dev, precision=prec, fp32_threshold=a.fp32_threshold, mixed_policy=a.mixed_policy)
         tune = be.autotune(pack, dens, reps=1, progress=_progress)
         _progress("batch %s: autotuned" % prec)
         replay = be.capture(pack, dens)
@@ -201,7 +201,7 @@ def bench_serve(a):
         reqs = poisson_requests(dens, a.iterations, rate, seed=7,
                                 perturb=lambda D0, m, it: configs.serve_iteration_density(D0, m, it))
         srv = FockServer(mols, dev=dev, hbm_budget=64e9, max_batch=32, precision="mixed",
-                         fp32_threshold=a.fp32_threshold)
+                         fp32_threshold=a.fp32_threshold, mixed_policy=a.mixed_policy)
         st = srv.serve(reqs, progress=_progress)
         d = st.as_dict()
         d["rate"] = rate
@@ -228,6 +228,9 @@ def main():
     ap.add_argument("--precision", choices=("both", "fp64", "mixed"), default="both")
     ap.add_argument("--fp32-threshold", type=float, default=1e-7,
                     help="Schwarz x |D| bound below which quartets run the VRR/HRR in FP32")
+    ap.add_argument("--mixed-policy", choices=("measured", "all"), default="measured",
+                    help="mixed precision: launch _mixed kernels only for the classes of "
+                         "vlxmi/gen/mixed_classes.json (measured faster) or for every class")
     ap.add_argument("--mode", choices=("fock", "batch", "serve"), default="fock",
                     help="fock: lattice64 J+K build (headline); batch: many molecules per launch; serve: request server")
     ap.add_argument("--molecules", type=int, default=None, help="batch/serve: number of molecules")
@@ -261,7 +264,7 @@ def main():
         del src, dst
     mol, D = configs.make_workload(a.workload)
     t0 = time.time()
-    eng = FockEngine(mol, fp32_threshold=a.fp32_threshold)
+    eng = FockEngine(mol, fp32_threshold=a.fp32_threshold, mixed_policy=a.mixed_policy)
     torch.cuda.synchronize()
     setup_s = time.time() - t0
     _progress("fock: %s lists and device data ready (%.1f s)" % (a.workload, setup_s))
@@ -383,7 +386,8 @@ def main():
         "data": "synthetic",
         "config": {"workload": WORKLOAD_TEXT.get(a.workload, a.workload) % {"nbf": mol.nbf, "ncart": mol.naos_cart,
                                                                              "nocc": mol.nbf // 4},
-                   "precision": "mixed (fp32 threshold %.0e)" % a.fp32_threshold, "nbf": mol.nbf,
+                   "precision": "mixed (fp32 threshold %.0e, %s classes)" % (a.fp32_threshold, a.mixed_policy),
+                   "nbf": mol.nbf,
                    "naos_cart": mol.naos_cart},
         "roofline": {"bound": "valu", "kernel": dom_m, "achieved": W[dom_m][1] / dom_mt / 1e12,
                      "peak": PEAK_FP64_VALU / 1e12, "unit": "TFLOP/s", "frac": W[dom_m][1] / dom_mt / PEAK_FP64_VALU,

@@ -83,3 +83,18 @@ def test_density_order_table():
             assert (lj, jl) == ((True, False) if cls in tab else (False, True)), (pre, cls)
     for cls in tab:
         assert cls in kc
+
+
+def test_mixed_policy_table():
+    from vlxmi import policy
+    tab = policy.mixed_classes()
+    assert set(tab) == {"J", "K"} and tab["J"] and tab["K"]
+    assert tab["J"] <= {ab + cd for ab, cd in j_classes()}
+    assert tab["K"] <= set(k_classes())
+    cj, ck = sorted(tab["J"])[0], sorted(tab["K"])[0]
+    assert policy.use_mixed("J", cj, "mixed") and policy.use_mixed("K", ck, "mixed")
+    assert not policy.use_mixed("J", cj, "fp64")
+    assert not policy.use_mixed("J", "ssss", "mixed") or "ssss" in tab["J"]
+    assert policy.use_mixed("K", "ssss", "mixed", "all")
+    with pytest.raises(ValueError):
+        policy.use_mixed("K", "ssss", "mixed", "some")

@@ -108,13 +108,14 @@ def test_autotune_keeps_results(dev, mols, pack):
     pk.tuning = {}
 
 
-def test_batched_mixed_error(dev, mols, pack):
+@pytest.mark.parametrize("policy", ["all", "measured"])
+def test_batched_mixed_error(dev, mols, pack, policy):
     from vlxmi.batch import BatchEngine
     entries, pk = pack
-    _say("case: batched mixed precision vs FP64")
+    _say("case: batched mixed precision vs FP64, policy %s" % policy)
     dens = [D for _, D in mols]
     J64, K64 = BatchEngine(dev).build(pk, dens)
-    Jm, Km = BatchEngine(dev, precision="mixed").build(pk, dens)
+    Jm, Km = BatchEngine(dev, precision="mixed", mixed_policy=policy).build(pk, dens)
     for m, D in enumerate(dens):
         Dt = dev.tensor(D)
         dEJ = 0.5 * float((Dt * (Jm[m] - J64[m])).sum().item())

@@ -20,12 +20,15 @@ def dev():
     return Device()
 
 
+@pytest.mark.parametrize("policy", ["all", "measured"])
 @pytest.mark.parametrize("wl", ["small8", "lattice4"])
-def test_mixed_vs_oracle(dev, wl):
+def test_mixed_vs_oracle(dev, wl, policy):
+    """policy "all": every class runs its _mixed kernel; "measured": only the
+    classes of vlxmi/gen/mixed_classes.json do (the engines' default)."""
     from vlxmi.fock import FockEngine
-    print("case: mixed precision %s" % wl, flush=True)
+    print("case: mixed precision %s, policy %s" % (wl, policy), flush=True)
     mol, D = configs.make_workload(wl)
-    eng = FockEngine(mol, dev=dev, precision="mixed")
+    eng = FockEngine(mol, dev=dev, precision="mixed", mixed_policy=policy)
     J, K = eng.build(D)
     J, K = J.cpu().numpy(), K.cpu().numpy()
     Dc = U.cart_density(mol, D)

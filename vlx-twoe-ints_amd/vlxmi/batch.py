@@ -28,6 +28,7 @@ import numpy as np
 
 from . import pairs as P
 from .basis import NCART
+from .policy import use_mixed
 from .runtime import Device, check, lib
 
 LV = P.LVAL
@@ -273,13 +274,16 @@ class Pack:
 
 
 class BatchEngine:
-    def __init__(self, dev: Device = None, thr=1e-12, precision="fp64", fp32_threshold=1e-6, omega=0.0):
+    def __init__(self, dev: Device = None, thr=1e-12, precision="fp64", fp32_threshold=1e-6, omega=0.0,
+                 mixed_policy="measured"):
         self.dev = dev or Device()
         self.torch = self.dev.torch
         self.thr = float(thr)
         self.precision = precision
         self.fp32_threshold = float(fp32_threshold)
         self.omega = float(omega)
+        # which classes launch their _mixed kernel in mixed precision (vlxmi.policy)
+        self.mixed_policy = mixed_policy
         self.timer = None
 
     # ------------------------------------------------------------------ helpers
@@ -346,7 +350,6 @@ class BatchEngine:
         kets = pack.kets
         matJ, matK = pack.matJ, pack.matK
         maxD = pack.maxD
-        sfx = "_batched" + ("_mixed" if self.precision == "mixed" else "")
         for buf in (pack.jbuf, pack.kbuf, pack.Jflat, pack.Kflat):
             zero_(self.dev, buf)
         st = self.dev.stream_ptr()
@@ -365,10 +368,11 @@ class BatchEngine:
             tun = pack.tuning.get(("J", cls), {})
             work = tun.get("work", pack.jwork_dev[(ab, cd)])
             args += [work.data_ptr(), len(items)]
-            if self.precision == "mixed":
+            mixed = use_mixed("J", cls, self.precision, self.mixed_policy)
+            if mixed:
                 args.append(self.fp32_threshold)
             args += [tun.get("wg", 0), st]
-            name = "computeCoulombFock" + cls.upper() + sfx
+            name = "computeCoulombFock" + cls.upper() + "_batched" + ("_mixed" if mixed else "")
             self._ev(name, getattr(L, "vlxmi_launch_" + name), *args)
         for cls in P.k_classes():
             a, b, c, d = cls
@@ -393,10 +397,11 @@ class BatchEngine:
             bdi = 3 * LV[b] + LV[d]
             args += [blk_list.data_ptr(), ik["nblk"], ik["blk_mol"].data_ptr(), maxD[bdi].data_ptr(),
                      pack.mol_D_off.data_ptr(), pack.mol_naos.data_ptr(), pack.mol_ao_off.data_ptr()]
-            if self.precision == "mixed":
+            mixed = use_mixed("K", cls, self.precision, self.mixed_policy)
+            if mixed:
                 args.append(self.fp32_threshold)
             args += [tun.get("wg", 0), st]
-            name = "computeExchangeFock" + cls.upper() + sfx
+            name = "computeExchangeFock" + cls.upper() + "_batched" + ("_mixed" if mixed else "")
             self._ev(name, getattr(L, "vlxmi_launch_" + name), *args)
         pack.jasm.run(pack.jbuf, pack.Jflat)
         pack.kasm.run(pack.kbuf, pack.Kflat)

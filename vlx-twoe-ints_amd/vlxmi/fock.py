@@ -28,12 +28,13 @@ import numpy as np
 
 from . import pairs as P
 from .basis import NCART
+from .policy import use_mixed
 from .runtime import Device, check, lib
 
 
 class FockEngine:
     def __init__(self, mol, thr=1e-12, dev=None, d_bound=100.0, omega=0.0, target_waves=2048,
-                 precision="fp64", fp32_threshold=1e-6):
+                 precision="fp64", fp32_threshold=1e-6, mixed_policy="measured"):
         self.dev = dev or Device()
         t = self.dev.torch
         self.torch = t
@@ -48,6 +49,8 @@ class FockEngine:
         assert precision in ("fp64", "mixed")
         self.precision = precision
         self.fp32_threshold = float(fp32_threshold)
+        # which classes launch their _mixed kernel in mixed precision (vlxmi.policy)
+        self.mixed_policy = mixed_policy
         t0 = time.time()
         self.fl = P.build_lists(mol, self.dev.schwarz, thr=thr, d_bound=d_bound)
         self.setup_lists_s = time.time() - t0
@@ -209,8 +212,9 @@ class FockEngine:
                      ket["first"].data_ptr(), ket["second"].data_ptr(), ket["pair_data"].data_ptr(), ket["count"],
                      self.dev.boys_table.data_ptr(), self.dev.boys_ft.data_ptr(), self.thr]
             split = self._ket_split(bra["nblk"], self.jl[cd]["nblk"])
-            name = "computeCoulombFock" + cls.upper() + ("_mixed" if self.precision == "mixed" else "")
-            if self.precision == "mixed":
+            mixed = use_mixed("J", cls, self.precision, self.mixed_policy)
+            name = "computeCoulombFock" + cls.upper() + ("_mixed" if mixed else "")
+            if mixed:
                 args.append(self.fp32_threshold)
             fn = getattr(L, "vlxmi_launch_" + name)
             ev = self._ev_start()
@@ -253,8 +257,9 @@ class FockEngine:
                 if cd != ab:
                     args.append(Cc[nm].data_ptr())
             args += [self.dev.boys_table.data_ptr(), self.dev.boys_ft.data_ptr(), self.omega, self.thr]
-            name = "computeExchangeFock" + cls.upper() + ("_mixed" if self.precision == "mixed" else "")
-            if self.precision == "mixed":
+            mixed = use_mixed("K", cls, self.precision, self.mixed_policy)
+            name = "computeExchangeFock" + cls.upper() + ("_mixed" if mixed else "")
+            if mixed:
                 args.append(self.fp32_threshold)
             fn = getattr(L, "vlxmi_launch_" + name)
             ev = self._ev_start()

@@ -66,7 +66,7 @@ class ServeStats:
 class FockServer:
     def __init__(self, molecules, dev: Device = None, hbm_budget=16e9, max_batch=32, precision="mixed",
                  thr=1e-12, fp32_threshold=1e-6, keep_results=False, entry_factory=None, batch_runner=None,
-                 synchronize=None):
+                 synchronize=None, mixed_policy="measured"):
         """``entry_factory(mol) -> entry with .nbytes`` and ``batch_runner(entries,
         densities) -> (Js, Ks)`` default to the GPU MolEntry / BatchEngine; they are
         injectable so the scheduling logic can be exercised without a GPU."""
@@ -75,7 +75,8 @@ class FockServer:
         self.max_batch = int(max_batch)
         if entry_factory is None or batch_runner is None:
             self.dev = dev or Device()
-            self.engine = BatchEngine(self.dev, thr=thr, precision=precision, fp32_threshold=fp32_threshold)
+            self.engine = BatchEngine(self.dev, thr=thr, precision=precision, fp32_threshold=fp32_threshold,
+                                      mixed_policy=mixed_policy)
         else:
             self.dev = dev
             self.engine = None

@@ -33,7 +33,7 @@ from .runtime import Device, check, lib
 
 
 class FockEngine:
-    def __init__(self, mol, thr=1e-12, dev=None, d_bound=100.0, omega=0.0, target_waves=2048,
+    def __init__(self, mol, thr=1e-12, dev=None, d_bound=100.0, omega=0.0, target_waves=32768,
                  precision="fp64", fp32_threshold=1e-6, mixed_policy="measured"):
         self.dev = dev or Device()
         t = self.dev.torch
@@ -185,6 +185,11 @@ class FockEngine:
         self.timer.append((name, e0, e1))
 
     def _ket_split(self, nbra_blocks, nket_blocks):
+        """Ket-list chunks per Coulomb launch: enough workgroups (one wavefront
+        each) for about ``target_waves`` waves, at least 64 ket blocks per
+        chunk.  The light classes wait on the Boys-table gathers; 32768 waves
+        (128 per CU over the launch) hide that latency where 2048 did not
+        (lattice64 J build 1.40 s -> 1.11 s)."""
         tiles = max(1, (nbra_blocks + 63) // 64)
         s = max(1, -(-self.target_waves // tiles))
         return int(min(s, max(1, nket_blocks // 64)))

@@ -26,6 +26,7 @@ for _p in (ROOT, os.path.join(ROOT, "vlx-twoe-ints_amd")):
 
 PEAK_FP64_VALU = 78.6e12     # MI355X FP64 vector, FLOP/s
 PEAK_HBM = 8.0e12            # bytes/s
+TRAFFIC_TABLE = os.path.join(ROOT, "profiles", "hbm_traffic_lattice64.json")
 
 
 _T0 = time.time()
@@ -356,15 +357,25 @@ def main():
         with open(a.per_class_out, "w") as f:
             json.dump(rows, f, indent=1, sort_keys=True)
 
+    traffic_src = None
     if a.pmc_dir:
         from vlxmi import profsum
         fe = os.path.join(a.pmc_dir, "fetch", "run_counter_collection.csv")
         wr = os.path.join(a.pmc_dir, "write", "run_counter_collection.csv")
         kf, kw = profsum.copy_calibration(fe, wr, 512 << 20)
-        fr = [v.get("FETCH_SIZE", 0) for k, v in profsum.counters_from(fe).items() if k[1] == dom_m.replace("_mixed", "")]
-        wrs = [v.get("WRITE_SIZE", 0) for k, v in profsum.counters_from(wr).items() if k[1] == dom_m.replace("_mixed", "")]
+        fr = [v.get("FETCH_SIZE", 0) for k, v in profsum.counters_from(fe).items() if k[1] == dom_m]
+        wrs = [v.get("WRITE_SIZE", 0) for k, v in profsum.counters_from(wr).items() if k[1] == dom_m]
         if fr and wrs and kf and kw:
             traffic_bytes = (sum(fr) / len(fr)) * kf + (sum(wrs) / len(wrs)) * kw
+            traffic_src = "FETCH_SIZE/WRITE_SIZE counter runs under %s, copy-calibrated" % a.pmc_dir
+    elif a.workload == "lattice64" and os.path.exists(TRAFFIC_TABLE):
+        # the committed counter runs of this same command (rocprofv3 --pmc
+        # FETCH_SIZE, then WRITE_SIZE; bytes per counter unit from a 512 MiB copy)
+        with open(TRAFFIC_TABLE) as f:
+            tk = json.load(f)["kernels"].get(dom_m)
+        if tk:
+            traffic_bytes = tk["read"] + tk["write"]
+            traffic_src = os.path.relpath(TRAFFIC_TABLE, ROOT) + " (FETCH_SIZE/WRITE_SIZE counter runs, copy-calibrated)"
     cpu = None
     if not a.no_cpu:
         _progress("fock: CPU oracle baseline (bounded sample)")
@@ -391,7 +402,8 @@ def main():
                    "naos_cart": mol.naos_cart},
         "roofline": {"bound": "valu", "kernel": dom_m, "achieved": W[dom_m][1] / dom_mt / 1e12,
                      "peak": PEAK_FP64_VALU / 1e12, "unit": "TFLOP/s", "frac": W[dom_m][1] / dom_mt / PEAK_FP64_VALU,
-                     "traffic": traffic_bytes, "algorithmic_flop_per_launch": W[dom_m][1],
+                     "traffic": traffic_bytes, "traffic_source": traffic_src,
+                     "algorithmic_flop_per_launch": W[dom_m][1],
                      "quartets_per_launch": W[dom_m][0], "avg_launch_s": dom_mt,
                      "fp64_path": {"kernel": dom, "achieved": achieved / 1e12, "frac": achieved / PEAK_FP64_VALU,
                                    "avg_launch_s": dom_t}},

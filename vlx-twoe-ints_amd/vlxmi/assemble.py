@@ -35,6 +35,23 @@ class Assembler:
         # outputs that receive no contribution stay as the caller zeroed them
         self.covers_all = self.nseg == self.nout
 
+    @classmethod
+    def from_device(cls, dev, perm, seg_start, seg_out, nout):
+        """A plan already sorted: ``perm`` value indices in plan order,
+        ``seg_start`` (nseg + 1 entries) where each output's run begins and
+        ends, ``seg_out`` the output of each run (device int64 tensors)."""
+        t = dev.torch
+        assert perm.numel() < 2 ** 31 and nout < 2 ** 31
+        self = cls.__new__(cls)
+        self.dev = dev
+        self.nseg = int(seg_out.numel())
+        self.nout = int(nout)
+        self.perm = perm.to(t.int32).contiguous()
+        self.seg_off = seg_start.to(t.int32).contiguous()
+        self.seg_out = seg_out.to(t.int32).contiguous()
+        self.covers_all = self.nseg == self.nout
+        return self
+
     def run(self, values, out):
         """out[...] = assembled sums (elements without contributions untouched)."""
         check(lib().vlxmi_segment_sum(values.data_ptr(), self.perm.data_ptr(), self.seg_off.data_ptr(),

@@ -27,6 +27,7 @@ import time
 import numpy as np
 
 from . import pairs as P
+from .assemble import Assembler
 from .basis import NCART
 from .policy import use_mixed
 from .runtime import Device, check, lib
@@ -54,31 +55,17 @@ class MolEntry:
         n = self.naos
         self.jl = {}
         for pc, jl in fl.jlists.items():
-            use = ~jl.lower_diag
-            off = (jl.first != jl.second) if jl.la == jl.lb else np.ones(jl.count, dtype=bool)
             self.jl[pc] = dict(first=d(jl.first), second=d(jl.second), pair_data=d(jl.pair_data),
                                mat_Q=d(jl.mat_Q), Didx=d((jl.ao_first * n + jl.ao_second).astype(np.int64)),
-                               keep=d((~jl.lower_diag).astype(np.float64)), nblk=jl.nblk, count=jl.count,
-                               sel1=d(np.nonzero(use)[0].astype(np.int64)),
-                               out1=d((jl.ao_first * n + jl.ao_second)[use].astype(np.int64)),
-                               sel2=d(np.nonzero(use & off)[0].astype(np.int64)),
-                               out2=d((jl.ao_second * n + jl.ao_first)[use & off].astype(np.int64)))
+                               keep=d((~jl.lower_diag).astype(np.float64)), nblk=jl.nblk, count=jl.count)
         self.kc = {}
         for pc, cs in fl.kcsr.items():
             self.kc[pc] = dict(displs=d(cs.displs), counts=d(cs.counts), Q=d(cs.Q), D_inds=d(cs.D_inds),
                                pair_data=d(cs.pair_data), nnz=int(cs.Q.size), nblkcsr=int(cs.blk_max.size))
         self.ik = {}
         for pc, ik in fl.iklists.items():
-            la, lc = ik.la, ik.lc
-            na, nc = NCART[la], NCART[lc]
-            ai = fl.prims[la].aoinds[ik.i % na, ik.i // na].astype(np.int64)
-            ak = fl.prims[lc].aoinds[ik.k % nc, ik.k // nc].astype(np.int64)
-            if la != lc:
-                sel2 = np.arange(ik.i.size)
-            else:
-                sel2 = np.nonzero(np.repeat(ik.I, na * nc) != np.repeat(ik.K, na * nc))[0]
-            self.ik[pc] = dict(i=d(ik.i), k=d(ik.k), nblk=ik.nblk, count=int(ik.i.size),
-                               out1=d(ai * n + ak), sel2=d(sel2.astype(np.int64)), out2=d(ak[sel2] * n + ai[sel2]))
+            self.ik[pc] = dict(i=d(ik.i), k=d(ik.k), nblk=ik.nblk, count=int(ik.i.size))
+        self._assembly_plans(fl)
         # AO type of every Cartesian AO (for the per-molecule maxD_bd table)
         aotype = np.zeros(n, dtype=np.int64)
         for p in fl.prims:
@@ -87,9 +74,62 @@ class MolEntry:
         self.pair_type = d((3 * aotype[:, None] + aotype[None, :]).ravel())
         self.nbytes = self._nbytes()
 
+    def _assembly_plans(self, fl):
+        """Sorted assembly plans of this molecule alone (value index -> AO
+        element, ordered by output then value index), computed once.  Value
+        indices are in the molecule's own layout (pair classes in PAIR_CLASSES
+        order, each class's entries consecutive); ``pcid`` records each entry's
+        pair class so a pack can move it to the class's slice of the batch
+        buffer.  Concatenating these plans with offsets gives the batch plan in
+        the same per-output order, so a molecule's sums are the same in any batch."""
+        n = self.naos
+        pcs = list(P.PAIR_CLASSES)
+        plans = {}
+        for kind in ("J", "K"):
+            vi, oi, pi = [], [], []
+            off = 0
+            for ipc, pc in enumerate(pcs):
+                if kind == "J":
+                    if pc not in fl.jlists or fl.jlists[pc].nblk == 0:
+                        continue
+                    jl = fl.jlists[pc]
+                    use = ~jl.lower_diag
+                    offd = (jl.first != jl.second) if jl.la == jl.lb else np.ones(jl.count, dtype=bool)
+                    sel1, sel2 = np.nonzero(use)[0], np.nonzero(use & offd)[0]
+                    out1 = (jl.ao_first * n + jl.ao_second)[sel1]
+                    out2 = (jl.ao_second * n + jl.ao_first)[sel2]
+                    cnt = jl.count
+                else:
+                    if pc not in fl.iklists or fl.iklists[pc].nblk == 0:
+                        continue
+                    ik = fl.iklists[pc]
+                    na, nc = NCART[ik.la], NCART[ik.lc]
+                    ai = fl.prims[ik.la].aoinds[ik.i % na, ik.i // na].astype(np.int64)
+                    ak = fl.prims[ik.lc].aoinds[ik.k % nc, ik.k // nc].astype(np.int64)
+                    sel1 = np.arange(ik.i.size)
+                    sel2 = (sel1 if ik.la != ik.lc
+                            else np.nonzero(np.repeat(ik.I, na * nc) != np.repeat(ik.K, na * nc))[0])
+                    out1, out2 = ai * n + ak, ak[sel2] * n + ai[sel2]
+                    cnt = int(ik.i.size)
+                vi += [sel1 + off, sel2 + off]
+                oi += [out1, out2]
+                pi += [np.full(sel1.size + sel2.size, ipc, dtype=np.uint8)]
+                off += cnt
+            vi = np.concatenate(vi).astype(np.int64) if vi else np.zeros(0, np.int64)
+            oi = np.concatenate(oi).astype(np.int64) if oi else np.zeros(0, np.int64)
+            pi = np.concatenate(pi) if pi else np.zeros(0, np.uint8)
+            order = np.lexsort((vi, oi))
+            vo, oo, po = vi[order], oi[order], pi[order]
+            starts = np.nonzero(np.r_[True, oo[1:] != oo[:-1]])[0] if oo.size else np.zeros(0, np.int64)
+            d = self.dev.tensor
+            plans[kind] = dict(vo=d(vo.astype(np.int32)), pcid=d(po.astype(np.int32)),
+                               seg_start=d(starts.astype(np.int32)), seg_out=d(oo[starts].astype(np.int32)),
+                               nval=int(vo.size), nseg=int(starts.size))
+        self.jplan, self.kplan = plans["J"], plans["K"]
+
     def _nbytes(self):
         tot = 0
-        for v in list(self.jl.values()) + list(self.kc.values()) + list(self.ik.values()):
+        for v in list(self.jl.values()) + list(self.kc.values()) + list(self.ik.values()) + [self.jplan, self.kplan]:
             for x in v.values():
                 if hasattr(x, "element_size"):
                     tot += x.element_size() * x.numel()
@@ -157,13 +197,9 @@ class Pack:
             second = t.cat([E[m].jl[pc]["second"] + self.poff[lb][m] * nb for m in ms]).contiguous()
             Didx = t.cat([E[m].jl[pc]["Didx"] + D_off[m] for m in ms]).contiguous()
             blk_mol = t.cat([t.full((E[m].jl[pc]["nblk"],), m, dtype=t.int64, device=dv) for m in ms])
-            sel1 = t.cat([E[m].jl[pc]["sel1"] + eoff[m] for m in ms])
-            out1 = t.cat([E[m].jl[pc]["out1"] + D_off[m] for m in ms])
-            sel2 = t.cat([E[m].jl[pc]["sel2"] + eoff[m] for m in ms])
-            out2 = t.cat([E[m].jl[pc]["out2"] + D_off[m] for m in ms])
             self.jl[pc] = dict(first=first, second=second, pair_data=g("pair_data"), mat_Q=g("mat_Q"), Didx=Didx,
-                               keep=g("keep"), blk_mol=blk_mol, nblk=b, count=c, nab=na * nb, boff=boff,
-                               nblk_m={m: E[m].jl[pc]["nblk"] for m in ms}, sel1=sel1, out1=out1, sel2=sel2, out2=out2)
+                               keep=g("keep"), blk_mol=blk_mol, nblk=b, count=c, nab=na * nb, boff=boff, eoff=eoff,
+                               nblk_m={m: E[m].jl[pc]["nblk"] for m in ms})
         # J work lists per class
         self.jwork = {}
         for ab, cd in P.j_classes():
@@ -175,13 +211,12 @@ class Pack:
                 if m not in Cc["nblk_m"]:
                     continue
                 kb, ke = Cc["boff"][m], Cc["boff"][m] + Cc["nblk_m"][m]
-                for t0 in range(0, nbra, 64):
-                    b0 = A["boff"][m] + t0
-                    b1 = min(A["boff"][m] + nbra, b0 + 64)
-                    items.append((b0, b1, kb, ke))
-                    cost.append((b1 - b0) * (ke - kb))
+                b0 = A["boff"][m] + np.arange(0, nbra, 64, dtype=np.int64)
+                b1 = np.minimum(A["boff"][m] + nbra, b0 + 64)
+                items.append(np.stack([b0, b1, np.full_like(b0, kb), np.full_like(b0, ke)], axis=1))
+                cost.append((b1 - b0).astype(np.float64) * (ke - kb))
             if items:
-                self.jwork[(ab, cd)] = (np.array(items, dtype=np.uint32), np.array(cost, dtype=np.float64))
+                self.jwork[(ab, cd)] = (np.concatenate(items).astype(np.uint32), np.concatenate(cost))
         # K CSR
         self.kc = {}
         for la in range(3):
@@ -226,10 +261,7 @@ class Pack:
                 i=t.cat([E[m].ik[pc]["i"] + self.poff[la][m] * na for m in ms]).contiguous(),
                 k=t.cat([E[m].ik[pc]["k"] + self.poff[lc][m] * nc for m in ms]).contiguous(),
                 blk_mol=t.cat([t.full((E[m].ik[pc]["nblk"],), m, dtype=i32, device=dv) for m in ms]).contiguous(),
-                nblk=sum(E[m].ik[pc]["nblk"] for m in ms), count=o3,
-                out1=t.cat([E[m].ik[pc]["out1"] + D_off[m] for m in ms]),
-                sel2=t.cat([E[m].ik[pc]["sel2"] + eoff[m] for m in ms]),
-                out2=t.cat([E[m].ik[pc]["out2"] + D_off[m] for m in ms]))
+                nblk=sum(E[m].ik[pc]["nblk"] for m in ms), count=o3, eoff=eoff)
             self.ik[pc]["blk_list"] = t.arange(self.ik[pc]["nblk"], dtype=i32, device=dv)
         self.mol_D_off = t.tensor(D_off, dtype=t.int64, device=dv)
         self.mol_naos = t.tensor([e.naos for e in E], dtype=i32, device=dv)
@@ -238,7 +270,6 @@ class Pack:
         self.jwork_dev = {k: dev.tensor(v[0].ravel()) for k, v in self.jwork.items()}
         self.tuning = {}
         # static per-class output buffers (views of one buffer) and assembly plans
-        from .assemble import Assembler
         self.j_order = [pc for pc in P.PAIR_CLASSES if pc in self.jl]
         self.k_order = [pc for pc in P.PAIR_CLASSES if pc in self.ik]
         joff, koff = {}, {}
@@ -254,15 +285,8 @@ class Pack:
         self.kbuf = t.zeros(max(o, 1), dtype=t.float64, device=dv)
         self.matJ = {pc: self.jbuf[joff[pc]: joff[pc] + self.jl[pc]["count"]] for pc in self.j_order}
         self.matK = {pc: self.kbuf[koff[pc]: koff[pc] + self.ik[pc]["count"]] for pc in self.k_order}
-        vi = t.cat([t.cat([self.jl[pc]["sel1"], self.jl[pc]["sel2"]]) + joff[pc] for pc in self.j_order])
-        oi = t.cat([t.cat([self.jl[pc]["out1"], self.jl[pc]["out2"]]) for pc in self.j_order])
-        self.jasm = Assembler(dev, vi.cpu().numpy(), oi.cpu().numpy(), self.Dsize)
-        vk, ok = [], []
-        for pc in self.k_order:
-            ik = self.ik[pc]
-            vk += [t.arange(ik["count"], device=dv) + koff[pc], ik["sel2"] + koff[pc]]
-            ok += [ik["out1"], ik["out2"]]
-        self.kasm = Assembler(dev, t.cat(vk).cpu().numpy(), t.cat(ok).cpu().numpy(), self.Dsize)
+        self.jasm = _pack_assembler(self, dev, joff, "jl", "jplan")
+        self.kasm = _pack_assembler(self, dev, koff, "ik", "kplan")
         self.Jflat = t.zeros(self.Dsize, dtype=t.float64, device=dv)
         self.Kflat = t.zeros(self.Dsize, dtype=t.float64, device=dv)
         # static density-dependent inputs (filled by BatchEngine.prepare)
@@ -271,6 +295,41 @@ class Pack:
                               pair_data=t.zeros_like(j["pair_data"]), mat_Q=t.zeros_like(j["mat_Q"]),
                               mat_D=t.zeros_like(j["pair_data"]), count=j["count"]) for pc, j in self.jl.items()}
         self.maxD = t.zeros((9, max(nm, 1)), dtype=t.float64, device=dv)
+
+
+def _pack_assembler(pack, dev, class_off, lists, plan_attr):
+    """The batch assembly plan: every molecule's own sorted plan, its value
+    indices moved to the batch buffer (class offset + the molecule's offset in
+    the class slice - the class offset in the molecule's own layout), its
+    outputs to the molecule's density slot.  Device-side concatenation."""
+    t = dev.torch
+    dv = dev.device
+    pcs = list(P.PAIR_CLASSES)
+    perm, seg_start, seg_out = [], [], []
+    vbase = 0
+    for m, e in enumerate(pack.entries):
+        plan = getattr(e, plan_attr)
+        if plan["nval"] == 0:
+            continue
+        own = {}
+        o = 0
+        src = e.jl if lists == "jl" else e.ik
+        for pc in pcs:
+            if pc in src and src[pc]["nblk"] > 0:
+                own[pc] = o
+                o += src[pc]["count"]
+        delta = np.zeros(len(pcs), dtype=np.int64)
+        for pc, o in own.items():
+            delta[pcs.index(pc)] = class_off[pc] + getattr(pack, lists)[pc]["eoff"][m] - o
+        perm.append(plan["vo"].long() + t.as_tensor(delta, device=dv)[plan["pcid"].long()])
+        seg_start.append(plan["seg_start"].long() + vbase)
+        seg_out.append(plan["seg_out"].long() + pack.D_off[m])
+        vbase += plan["nval"]
+    if not perm:
+        z = t.zeros(0, dtype=t.int64, device=dv)
+        perm, seg_start, seg_out = [z], [z], [z]
+    starts = t.cat(seg_start + [t.tensor([vbase], dtype=t.int64, device=dv)])
+    return Assembler.from_device(dev, t.cat(perm), starts, t.cat(seg_out), pack.Dsize)
 
 
 class BatchEngine:

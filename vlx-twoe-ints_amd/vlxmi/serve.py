@@ -55,6 +55,9 @@ class ServeStats:
     max_resident: int
     evictions: int
     setups: int
+    setup_s: float = 0.0           # making molecules resident (lists, Schwarz, plans)
+    pack_s: float = 0.0            # concatenating resident molecules into batches
+    build_s: float = 0.0           # density preparation, launches and results (synchronised)
     latencies_ms: list = field(default_factory=list)
 
     def as_dict(self):
@@ -88,6 +91,7 @@ class FockServer:
         self.used = 0
         self.evictions = 0
         self.setups = 0
+        self.setup_s = self.pack_s = self.build_s = 0.0
         self.keep_results = keep_results
         self.batch_log = []
 
@@ -109,7 +113,9 @@ class FockServer:
         if mol_id in self.resident:
             self.resident.move_to_end(mol_id)
             return True
+        t0 = time.perf_counter()
         e = self.entry_factory(self.molecules[mol_id])
+        self.setup_s += time.perf_counter() - t0
         self.setups += 1
         if not self._evict_for(e.nbytes, busy):
             del e
@@ -120,7 +126,9 @@ class FockServer:
 
     # ------------------------------------------------------------ one batch
     def _gpu_batch(self, entries, densities):
+        t0 = time.perf_counter()
         pack = Pack(entries, self.dev)
+        self.pack_s += time.perf_counter() - t0
         return self.engine.build(pack, densities)
 
     def run_batch(self, reqs):
@@ -152,13 +160,17 @@ class FockServer:
                 continue
             batch, chosen = [], set()
             busy = set(r.mol_id for r in pending)
+            first_it = {}
+            for q in pending:
+                if q.iteration < first_it.get(q.mol_id, q.iteration + 1):
+                    first_it[q.mol_id] = q.iteration
             for r in list(pending):
                 if len(batch) >= self.max_batch:
                     break
                 if r.mol_id in chosen:
                     continue
                 # a molecule's earlier iteration still pending keeps order
-                if any(q.mol_id == r.mol_id and q.iteration < r.iteration for q in pending if q is not r):
+                if r.iteration > first_it[r.mol_id]:
                     continue
                 if not self.admit(r.mol_id, busy | chosen):
                     continue
@@ -168,8 +180,10 @@ class FockServer:
                 raise RuntimeError("no pending request fits the HBM budget (%.3g bytes)" % self.budget)
             for r in batch:
                 pending.remove(r)
+            tb = time.perf_counter()
             Js, Ks = self.run_batch(batch)
             self.synchronize()
+            self.build_s += time.perf_counter() - tb
             t_done = time.perf_counter() - t0
             for r, J, K in zip(batch, Js, Ks):
                 r.t_done = t_done
@@ -186,7 +200,8 @@ class FockServer:
         return ServeStats(n=len(reqs), wall_s=wall, builds_per_s=len(reqs) / wall,
                           p50_ms=float(np.percentile(lat, 50)), p99_ms=float(np.percentile(lat, 99)),
                           mean_batch=float(np.mean(batches)), max_resident=max_res,
-                          evictions=self.evictions, setups=self.setups, latencies_ms=lat.tolist())
+                          evictions=self.evictions, setups=self.setups, setup_s=self.setup_s,
+                          pack_s=self.pack_s, build_s=self.build_s - self.pack_s, latencies_ms=lat.tolist())
 
 
 def poisson_requests(mol_densities, iterations, rate, seed=0, perturb=None):

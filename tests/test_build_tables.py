@@ -98,3 +98,23 @@ def test_mixed_policy_table():
     assert policy.use_mixed("K", "ssss", "mixed", "all")
     with pytest.raises(ValueError):
         policy.use_mixed("K", "ssss", "mixed", "some")
+
+
+def test_mixed_fp32_passes():
+    """The FP32 part of a mixed kernel runs its own (never more) passes:
+    FP32 intermediates take half the registers of FP64 ones.  A heavy exchange
+    class whose FP32 recursion needs as many passes keeps one FP64-then-FP32
+    loop pair per pass; one that needs fewer runs its FP32 passes after all
+    FP64 passes."""
+    for cls in ("sddd", "ppdp", "pddd"):
+        la, lb, lc, ld = [emit.LV[x] for x in cls]
+        n64 = len(emit.plan_slices(la, lb, lc, ld, "Kheavy"))
+        n32 = len(emit.plan_slices_f32(la, lb, lc, ld, "Kheavy"))
+        assert 1 <= n32 <= n64
+        src = emit.emit_k(cls, mixed=True)
+        assert src.count("< fp32_threshold)) { lo") == n64
+        assert src.count("bool actf") == (n32 if n32 < n64 else 0)
+    assert len(emit.plan_slices_f32(0, 2, 2, 2, "Kheavy")) < len(emit.plan_slices(0, 2, 2, 2, "Kheavy"))
+    for cls in ("sddd", "pppd"):
+        la, lb, lc, ld = [emit.LV[x] for x in cls]
+        assert len(emit.plan_slices_f32(la, lb, lc, ld, "J")) < len(emit.plan_slices(la, lb, lc, ld, "J"))

@@ -176,6 +176,33 @@ def plan_slices(la, lb, lc, ld, kind="J", budget=REG_BUDGET):
     return best
 
 
+_SLICE32_CACHE = {}
+
+
+def plan_slices_f32(la, lb, lc, ld, kind="J", budget=REG_BUDGET):
+    """Passes for the FP32 recursion of a mixed kernel: as plan_slices, with
+    the recursion intermediates counted at half a double each (the
+    accumulators, thresholds and prefactors stay FP64)."""
+    key = (la, lb, lc, ld, kind, budget)
+    if key in _SLICE32_CACHE:
+        return _SLICE32_CACHE[key]
+    nc = (lc + 1) * (lc + 2) // 2
+    nd = (ld + 1) * (ld + 2) // 2
+    pers = persistent_doubles(kind, la, lb, lc, ld)
+    best = [[i] for i in range(nc)]
+    for nsl in range(1, nc + 1):
+        if nc % nsl:
+            continue
+        size = nc // nsl
+        groups = [list(range(k * size, (k + 1) * size)) for k in range(nsl)]
+        need = max(_live_estimate(la, lb, lc, ld, g_)[0] for g_ in groups) / 2.0 + pers + 2 * size * nd
+        if need <= budget:
+            best = groups
+            break
+    _SLICE32_CACHE[key] = best
+    return best
+
+
 def _dag_lines(dag):
     # the DAG refers to PA_/QC_/AB_/CD_ of the geometric type: rename to g*
     out = []
@@ -183,6 +210,23 @@ def _dag_lines(dag):
         ln = ln.replace("PA_", "gPA_").replace("QC_", "gQC_").replace("AB_", "gAB_").replace("CD_", "gCD_")
         out.append(ln)
     return out
+
+
+def ket_loop(ind, act, screen, bound, header, body, ty, check_lo, lo):
+    """One loop over the staged records from the current ``s`` to nt-1, the
+    recursion in type ``ty``; with ``check_lo`` it stops (setting ``lo``) at
+    the first record where every live lane's bound is below fp32_threshold."""
+    q = [ind + "for (; s < nt; ++s) {",
+         ind + "    const double* R = ket + s * REC;",
+         ind + "    if (%s && %s) %s = false;" % (act, screen, act),
+         ind + "    if (!__any(%s)) break;" % act]
+    if check_lo:
+        q.append(ind + "    if (__all(!%s || %s < fp32_threshold)) { %s = true; break; }" % (act, bound, lo))
+    q.append(ind + "    if (%s) {" % act)
+    q += [ind + "        " + x for x in header]
+    q += body(ty, ind + "        ")
+    q += [ind + "    }", ind + "}"]
+    return q
 
 
 def ket_loops(ind, act, screen, bound, header, body, mixed, lo):
@@ -198,17 +242,7 @@ def ket_loops(ind, act, screen, bound, header, body, mixed, lo):
     flag ``lo`` carries the switch to later tiles.  Each loop then holds one
     recursion body only."""
     def loop(ty, check_lo):
-        q = [ind + "for (; s < nt; ++s) {",
-             ind + "    const double* R = ket + s * REC;",
-             ind + "    if (%s && %s) %s = false;" % (act, screen, act),
-             ind + "    if (!__any(%s)) break;" % act]
-        if check_lo:
-            q.append(ind + "    if (__all(!%s || %s < fp32_threshold)) { %s = true; break; }" % (act, bound, lo))
-        q.append(ind + "    if (%s) {" % act)
-        q += [ind + "        " + x for x in header]
-        q += body(ty, ind + "        ")
-        q += [ind + "    }", ind + "}"]
-        return q
+        return ket_loop(ind, act, screen, bound, header, body, ty, check_lo, lo)
 
     o = [ind + "{", ind + "uint32_t s = 0;"]
     if mixed:
@@ -456,8 +490,10 @@ def emit_j(cls, mixed=False, batched=False, kt=None, layout="reg"):
 
     # J keeps its passes inside one ket loop (the bra-side state is small);
     # measured faster than one loop per pass
+    # the FP32 loop of a mixed kernel may run fewer passes (half-size values)
+    jsl32 = plan_slices_f32(la, lb, lc, ld, "J") if mixed else jsl
     o += ket_loops("        ", "active", "qmax * R[0] * (1.0 + 1e-13) <= thr", "qmax * R[0]", KET_HEADER,
-                   lambda ty, ind: jbody(ty, ind, jsl), mixed, "lo_mode")
+                   lambda ty, ind: jbody(ty, ind, jsl if ty == "double" else jsl32), mixed, "lo_mode")
     o.append("    }")
     o.append("    if (own) {")
     o.append("        const uint32_t e0 = bb * NAB;")
@@ -764,11 +800,24 @@ def emit_k(cls, mixed=False, batched=False, kt=None, layout="reg", dorder="jl"):
 
     kscreen = "qij * R[0] * maxD * (1.0 + 1e-13) <= thr"
     kbound = "qij * R[0] * maxD"
-    if heavy:
+    def pacc_io(group, store):
+        out = []
+        if lacc:
+            for ic in group:
+                for ia in range(na):
+                    t = ia * nc + ic
+                    if store:
+                        out.append("            lacc[%d * TILE_SIZE_K + lane] = pacc%d;" % (t, t))
+                    else:
+                        out.append("            double pacc%d = lacc[%d * TILE_SIZE_K + lane];" % (t, t))
+        return out
+
+    kslices32 = plan_slices_f32(la, lb, lc, ld, "Kheavy") if mixed else kslices
+    if heavy and len(kslices32) == len(kslices):
         # heavy classes: one loop over the tile per pass of ket-first-index
         # components, so that only one pass's recursion is live at a time; the
         # screening sequence is the same in every pass, so each pass stops at
-        # the same record
+        # the same record (mixed: each pass's FP64 loop, then its FP32 loop)
         o.append("            bool act_end = active;")
         if mixed:
             o.append("            bool lo_end = lo_mode;")
@@ -777,18 +826,10 @@ def emit_k(cls, mixed=False, batched=False, kt=None, layout="reg", dorder="jl"):
             o.append("            bool act%d = active;" % isl)
             if mixed:
                 o.append("            bool lo%d = lo_mode;" % isl)
-            if lacc:
-                for ic in group:
-                    for ia in range(na):
-                        t = ia * nc + ic
-                        o.append("            double pacc%d = lacc[%d * TILE_SIZE_K + lane];" % (t, t))
+            o += pacc_io(group, False)
             o += ket_loops("            ", "act%d" % isl, kscreen, kbound, KET_HEADER,
                            lambda ty, ind, g=group: kbody(ty, ind, [g]), mixed, "lo%d" % isl)
-            if lacc:
-                for ic in group:
-                    for ia in range(na):
-                        t = ia * nc + ic
-                        o.append("            lacc[%d * TILE_SIZE_K + lane] = pacc%d;" % (t, t))
+            o += pacc_io(group, True)
             o.append("            act_end = act%d;" % isl)
             if mixed:
                 o.append("            lo_end = lo%d;" % isl)
@@ -796,6 +837,41 @@ def emit_k(cls, mixed=False, batched=False, kt=None, layout="reg", dorder="jl"):
         o.append("            active = act_end;")
         if mixed:
             o.append("            lo_mode = lo_end;")
+    elif heavy:
+        # mixed heavy classes whose FP32 recursion fits in fewer passes: the
+        # FP64 passes (one loop each) run up to the first record where the
+        # whole wavefront is below fp32_threshold (the same record in every
+        # pass: identical screening), then the FP32 passes run from there
+        o.append("            uint32_t s_sw = 0;")
+        o.append("            bool act_sw = active, lo_sw = lo_mode;")
+        for isl, group in enumerate(kslices):
+            o.append("            {")
+            o.append("            bool act%d = active, lo%d = lo_mode;" % (isl, isl))
+            o += pacc_io(group, False)
+            o.append("            uint32_t s = 0;")
+            o.append("            if (!lo%d) {" % isl)
+            o += ["    " + x for x in ket_loop("            ", "act%d" % isl, kscreen, kbound, KET_HEADER,
+                                             lambda ty, ind, g=group: kbody(ty, ind, [g]), "double", True,
+                                             "lo%d" % isl)]
+            o.append("            }")
+            o += pacc_io(group, True)
+            o.append("            s_sw = s; act_sw = act%d; lo_sw = lo%d;" % (isl, isl))
+            o.append("            }")
+        o.append("            bool act_end = act_sw;")
+        o.append("            if (lo_sw) {")
+        for isl, group in enumerate(kslices32):
+            o.append("            {")
+            o.append("            bool actf%d = act_sw;" % isl)
+            o += pacc_io(group, False)
+            o.append("            uint32_t s = s_sw;")
+            o += ket_loop("            ", "actf%d" % isl, kscreen, kbound, KET_HEADER,
+                          lambda ty, ind, g=group: kbody(ty, ind, [g]), "float", False, "lo_sw")
+            o += pacc_io(group, True)
+            o.append("            act_end = actf%d;" % isl)
+            o.append("            }")
+        o.append("            }")
+        o.append("            active = act_end;")
+        o.append("            lo_mode = lo_sw;")
     else:
         o += ket_loops("            ", "active", kscreen, kbound, KET_HEADER,
                        lambda ty, ind: kbody(ty, ind, [list(range(nc))]), mixed, "lo_mode")

@@ -24,6 +24,11 @@ int vlxmi_boys_table_fill(double* table, double* ft);
 /* out[t*(L+1)+m] = F_m(T[t]), the device Boys function (test probe). */
 int vlxmi_boys_eval(const double* T, uint32_t n, uint32_t L, const double* table, double* out, void* stream);
 
+/* As vlxmi_boys_eval through the table's coarse rows copied to LDS, the
+ * evaluation some light-class kernels use (L = 0..4; test probe). */
+int vlxmi_boys_eval_coarse(const double* T, uint32_t n, uint32_t L, const double* table, double* out,
+                           void* stream);
+
 /* Human-readable text of a status code (static string). */
 const char* vlxmi_status_string(int code);
 

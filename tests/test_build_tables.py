@@ -49,11 +49,24 @@ def test_tile_table_entries():
 
 
 def test_sources_use_table_values():
+    """Kernels take their ket tile from the tile table, except those that keep
+    the coarse Boys rows in LDS (a fixed 16-record tile)."""
     tiles = B.load_tiles()
+    blds = B.load_boys_lds()
     srcs = B.sources()
-    for fn, kt in list(tiles.items())[:20]:
+    checked = 0
+    for fn, kt in tiles.items():
+        kind = "J" if fn.startswith("j") else "K"
+        cls = fn.split("_")[1].split(".")[0]
         m = re.search(r"constexpr int KT = (\d+)", srcs[fn])
-        assert m and int(m.group(1)) == kt
+        want = emit.LDS_BOYS_KT if (cls in blds[kind] and not fn.startswith("schwarz")) else kt
+        assert m and int(m.group(1)) == want, fn
+        checked += 1
+    assert checked == len(tiles)
+    for kind, pre in (("J", "j"), ("K", "k")):
+        for cls in blds[kind]:
+            s = srcs["%s_%s.hip" % (pre, cls)]
+            assert "vlxmi::boys_lds<" in s and "vlxmi::boys<" not in s and "__syncthreads();" in s
 
 
 def test_mixed_kernels_have_one_body_per_loop():

@@ -39,14 +39,20 @@ def test_header_is_current():
 
 
 def test_boys_table_fill_matches_oracle():
+    """Fine rows: F_0..F_14 and exp(-T) at T = 0.02 k, k = 0..2000 (T in [0, 40]);
+    then coarse rows: F_0..F_15 at T = 0.4 k, k = 0..100."""
     table, ft = runtime.boys_tables_host()
     t = table.reshape(-1, 16)
-    n = t.shape[0]
-    dt = 40.0 / (n - 1)            # the grid covers T in [0, 40]
-    for k in (0, 1, 37, n // 2, n - 2, n - 1):
+    nfine, ncoarse = 2001, 101
+    assert t.shape[0] == nfine + ncoarse
+    dt = 40.0 / (nfine - 1)
+    for k in (0, 1, 37, nfine // 2, nfine - 2, nfine - 1):
         F = oracle.boys(14, dt * k)
         assert np.max(np.abs(t[k, :15] - F) / F) < 2e-15
         assert abs(t[k, 15] - np.exp(-dt * k)) <= 1e-16 * max(1.0, np.exp(-dt * k))
+    for k in (0, 1, 2, 50, 99, 100):
+        F = oracle.boys(15, 0.4 * k)
+        assert np.max(np.abs(t[nfine + k] - F) / F) < 2e-15, k
     assert np.allclose(ft, 1.0 / (2 * np.arange(32) + 1))
 
 

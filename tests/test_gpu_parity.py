@@ -45,6 +45,11 @@ def test_boys_device(dev):
         ref = np.array([oracle.boys(L, t) for t in T])
         rel = np.max(np.abs(F - ref) / np.abs(ref))
         assert rel < 1e-14, (L, rel)
+        if L <= 4:
+            # the coarse rows in LDS (12-term Taylor) that some light-class kernels use
+            Fc = dev.boys_eval(T, L, coarse=True)
+            relc = np.max(np.abs(Fc - ref) / np.abs(ref))
+            assert relc < 1e-14, (L, "coarse", relc)
 
 
 def test_schwarz_device(small8):

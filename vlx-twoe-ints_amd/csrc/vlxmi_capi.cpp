@@ -13,7 +13,7 @@ extern "C" int vlxmi_version(void) { return 1; }
 extern "C" int vlxmi_boys_table_size(uint32_t* nrow, uint32_t* ncol, uint32_t* nft)
 {
     if (!nrow || !ncol || !nft) return VLXMI_ERR_ARG;
-    *nrow = VLX_BOYS_NROW;
+    *nrow = VLX_BOYS_NROW + VLX_BOYS_CROWS;
     *ncol = VLX_BOYS_NCOL;
     *nft = 32;
     return VLXMI_OK;
@@ -45,6 +45,11 @@ extern "C" int vlxmi_boys_table_fill(double* table, double* ft)
         for (int m = 0; m < VLX_BOYS_NCOL - 1; ++m) table[k * VLX_BOYS_NCOL + m] = (double)F[m];
         table[k * VLX_BOYS_NCOL + VLX_BOYS_NCOL - 1] = (double)expl(-T);
     }
+    for (int k = 0; k < VLX_BOYS_CROWS; ++k) {
+        const long double T = (long double)k * (long double)VLX_BOYS_CDT;
+        boys_series(VLX_BOYS_NCOL - 1, T, F);
+        for (int m = 0; m < VLX_BOYS_NCOL; ++m) table[VLX_BOYS_COARSE_OFF + k * VLX_BOYS_NCOL + m] = (double)F[m];
+    }
     for (int m = 0; m < 32; ++m) ft[m] = 1.0 / (2.0 * m + 1.0);
     return VLXMI_OK;
 }
@@ -70,6 +75,40 @@ __global__ void boys_probe_kernel(const double* __restrict__ T, uint32_t n, cons
     vlxmi::boys<L>(T[t], table, F);
 #pragma unroll
     for (int m = 0; m <= L; ++m) out[(size_t)t * (L + 1) + m] = F[m];
+}
+
+template <int L>
+__global__ void boys_coarse_probe_kernel(const double* __restrict__ T, uint32_t n,
+                                         const double* __restrict__ table, double* __restrict__ out)
+{
+    __shared__ double lt[VLX_BOYS_CROWS * VLX_BOYS_CTERMS];
+    vlxmi::boys_lds_fill<L>(table, lt);
+    __syncthreads();
+    const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n) return;
+    double F[L + 1];
+    vlxmi::boys_lds<L>(T[t], lt, F);
+#pragma unroll
+    for (int m = 0; m <= L; ++m) out[(size_t)t * (L + 1) + m] = F[m];
+}
+
+// As vlxmi_boys_eval, through the coarse rows copied to LDS (the evaluation
+// some light-class kernels use); L = 0..4.
+extern "C" int vlxmi_boys_eval_coarse(const double* T, uint32_t n, uint32_t L, const double* table, double* out,
+                                      void* stream)
+{
+    if (n == 0) return VLXMI_OK;
+    if (!T || !table || !out || L > 4) return VLXMI_ERR_ARG;
+    dim3 g((n + 63) / 64), b(64);
+    hipStream_t s = (hipStream_t)stream;
+    switch (L) {
+        case 0: hipLaunchKernelGGL(boys_coarse_probe_kernel<0>, g, b, 0, s, T, n, table, out); break;
+        case 1: hipLaunchKernelGGL(boys_coarse_probe_kernel<1>, g, b, 0, s, T, n, table, out); break;
+        case 2: hipLaunchKernelGGL(boys_coarse_probe_kernel<2>, g, b, 0, s, T, n, table, out); break;
+        case 3: hipLaunchKernelGGL(boys_coarse_probe_kernel<3>, g, b, 0, s, T, n, table, out); break;
+        default: hipLaunchKernelGGL(boys_coarse_probe_kernel<4>, g, b, 0, s, T, n, table, out); break;
+    }
+    return (int)hipGetLastError();
 }
 
 // out[t*(L+1)+m] = F_m(T[t]) for the device Boys function of order L (0..8).

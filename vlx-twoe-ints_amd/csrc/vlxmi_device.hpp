@@ -5,9 +5,10 @@
 // boys_func_table, boys_func_ft) (gen_coulomb.py:257-259); that function is not
 // in the reference tree, so the table format below is this project's own:
 //
-//   boys_func_table: VLX_BOYS_NROW rows x VLX_BOYS_NCOL doubles, row k holds
-//     F_0..F_14 at T_k = k * VLX_BOYS_DT (columns 0..14) and exp(-T_k)
-//     (column 15); one 128-byte row per grid point.
+//   boys_func_table: VLX_BOYS_NROW fine rows x VLX_BOYS_NCOL doubles, row k
+//     holds F_0..F_14 at T_k = k * VLX_BOYS_DT (columns 0..14) and exp(-T_k)
+//     (column 15); one 128-byte row per grid point; then VLX_BOYS_CROWS coarse
+//     rows (below).
 //   boys_func_ft: 1/(2m+1), m = 0..31 (the generated kernels fold these
 //     constants at compile time; the array is kept for the signature).
 //
@@ -25,6 +26,13 @@
 // For T >= VLX_BOYS_TMAX: F_0 = sqrt(pi/T)/2 (Newton-refined 1/sqrt) and upward recursion
 // F_{m+1} = ((2m+1) F_m - exp(-T)) / (2T); there exp(-T) < 5e-18 and the
 // single-precision hardware exponential is accurate enough for it.
+//
+// Coarse rows (after the fine table, VLX_BOYS_CROWS rows of VLX_BOYS_NCOL
+// doubles): F_0..F_15 at T_k = k * VLX_BOYS_CDT.  Kernels of a few light
+// classes copy the 12 columns F_L..F_{L+11} of these rows, scaled by 1/j!, into
+// LDS once per workgroup (9.7 KB) and evaluate F_L by a 12-term Taylor expansion
+// about the nearest coarse point, |T - T_k| <= 0.2, remainder below
+// 0.2^12 / 12! = 8.6e-18 (relative to F_L): no global gather per quartet.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -37,6 +45,11 @@
 #define VLX_BOYS_DT 0.02
 #define VLX_BOYS_INV_DT 50.0
 #define VLX_BOYS_TMAX 40.0
+#define VLX_BOYS_CROWS 101
+#define VLX_BOYS_CDT 0.4
+#define VLX_BOYS_INV_CDT 2.5
+#define VLX_BOYS_CTERMS 12
+#define VLX_BOYS_COARSE_OFF (VLX_BOYS_NROW * VLX_BOYS_NCOL)
 
 #define MATH_CONST_PI 3.14159265358979323846
 #define MATH_CONST_INV_PI 0.31830988618379067154
@@ -106,6 +119,52 @@ __device__ __forceinline__ void boys(double T, const double* __restrict__ table,
     } else {
         const double inv_T = rcp_nr(T);
         F[0] = (0.5 * 1.77245385090551602730) * rsq_nr(T);   // sqrt(pi/T) / 2
+        if (L > 0) {
+            const double et = (double)__expf(-(float)T);
+            const double h = 0.5 * inv_T;
+#pragma unroll
+            for (int m = 0; m < L; ++m) F[m + 1] = ((2.0 * m + 1.0) * F[m] - et) * h;
+        }
+    }
+}
+
+// The LDS copy of the coarse rows for order L: lt[k * 12 + j] = F_{L+j}(T_k) / j!.
+// Every thread of the workgroup calls it; the caller synchronises afterwards.
+template <int L>
+__device__ __forceinline__ void boys_lds_fill(const double* __restrict__ table, double* lt)
+{
+    static_assert(L + VLX_BOYS_CTERMS - 1 < VLX_BOYS_NCOL, "coarse rows hold F_0..F_15");
+    constexpr double inv_fact[VLX_BOYS_CTERMS] = {1.0, 1.0, 1.0 / 2, 1.0 / 6, 1.0 / 24, 1.0 / 120, 1.0 / 720,
+                                                  1.0 / 5040, 1.0 / 40320, 1.0 / 362880, 1.0 / 3628800,
+                                                  1.0 / 39916800};
+    for (int t = threadIdx.x; t < VLX_BOYS_CROWS * VLX_BOYS_CTERMS; t += blockDim.x) {
+        const int r = t / VLX_BOYS_CTERMS, j = t - r * VLX_BOYS_CTERMS;
+        lt[t] = table[VLX_BOYS_COARSE_OFF + r * VLX_BOYS_NCOL + L + j] * inv_fact[j];
+    }
+}
+
+// F_0..F_L from the LDS copy of the coarse rows (T < VLX_BOYS_TMAX), exp(-T)
+// and downward recursion; the T >= VLX_BOYS_TMAX branch is that of boys<L>.
+template <int L>
+__device__ __forceinline__ void boys_lds(double T, const double* lt, double* F)
+{
+    if (T < VLX_BOYS_TMAX) {
+        const int k = (int)(T * VLX_BOYS_INV_CDT + 0.5);
+        const double nd = (double)k * VLX_BOYS_CDT - T;   // -(T - T_k)
+        const double* row = lt + k * VLX_BOYS_CTERMS;
+        double s = row[VLX_BOYS_CTERMS - 1];
+#pragma unroll
+        for (int j = VLX_BOYS_CTERMS - 2; j >= 0; --j) s = fma(s, nd, row[j]);
+        F[L] = s;
+        if (L > 0) {
+            const double et = exp(-T);
+            const double T2 = 2.0 * T;
+#pragma unroll
+            for (int m = L - 1; m >= 0; --m) F[m] = fma(T2, F[m + 1], et) * (1.0 / (2.0 * m + 1.0));
+        }
+    } else {
+        const double inv_T = rcp_nr(T);
+        F[0] = (0.5 * 1.77245385090551602730) * rsq_nr(T);
         if (L > 0) {
             const double et = (double)__expf(-(float)T);
             const double h = 0.5 * inv_T;

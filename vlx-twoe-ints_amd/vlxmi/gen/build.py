@@ -55,6 +55,18 @@ J_LAYOUT_TABLE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "j_lay
 # exchange classes that read the density as D[l][j] ("lj", see emit.emit_k),
 # chosen from per-class timings of both orders on lattice64
 DORDER_TABLE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "k_dorder.json")
+# light classes that evaluate the Boys function from coarse rows kept in LDS
+# (see emit.lds_boys_setup), chosen from per-class timings on lattice64
+BOYS_LDS_TABLE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "boys_lds.json")
+
+
+def load_boys_lds():
+    """{"J": set of classes, "K": set of classes} using the LDS Boys rows."""
+    if os.path.exists(BOYS_LDS_TABLE):
+        with open(BOYS_LDS_TABLE) as f:
+            tab = json.load(f)
+        return {"J": set(tab.get("J", [])), "K": set(tab.get("K", []))}
+    return {"J": set(), "K": set()}
 LLVM = "/opt/rocm/llvm/bin"
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = "gfx950"
@@ -88,6 +100,7 @@ def sources(tiles=None, layouts=None):
     layouts = load_layouts() if layouts is None else layouts
     jlayouts = load_layouts(J_LAYOUT_TABLE)
     dorders = load_layouts(DORDER_TABLE)
+    blds = load_boys_lds()
     out = {}
     inc = '#include "vlxmi_device.hpp"\n\n'
     for ab, cd in j_classes():
@@ -95,13 +108,14 @@ def sources(tiles=None, layouts=None):
         for pre, kw in (("j", {}), ("jm", dict(mixed=True)), ("jb", dict(batched=True)),
                         ("jbm", dict(batched=True, mixed=True))):
             fn = "%s_%s.hip" % (pre, cls)
-            out[fn] = inc + emit.emit_j(cls, kt=tiles.get(fn), layout=jlayouts.get(cls, "reg"), **kw)
+            out[fn] = inc + emit.emit_j(cls, kt=tiles.get(fn), layout=jlayouts.get(cls, "reg"),
+                                        lds_boys=cls in blds["J"], **kw)
     for cls in k_classes():
         for pre, kw in (("k", {}), ("km", dict(mixed=True)), ("kb", dict(batched=True)),
                         ("kbm", dict(batched=True, mixed=True))):
             fn = "%s_%s.hip" % (pre, cls)
             out[fn] = inc + emit.emit_k(cls, kt=tiles.get(fn), layout=layouts.get(cls, "reg"),
-                                        dorder=dorders.get(cls, "jl"), **kw)
+                                        dorder=dorders.get(cls, "jl"), lds_boys=cls in blds["K"], **kw)
     for pc in PAIR_CLASSES:
         out["schwarz_%s.hip" % pc] = inc + emit.emit_schwarz(pc)
     return out

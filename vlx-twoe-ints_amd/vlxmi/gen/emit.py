@@ -37,6 +37,18 @@ KET_TILE = 64
 # LDS a single-wavefront workgroup may use without lowering the occupancy of a
 # kernel that needs all 512 registers (160 KB per CU / 4 SIMDs)
 LDS_PER_WAVE_FULL_REGS = 40960
+# ket tile of the kernels that keep the coarse Boys rows in LDS (9.7 KB)
+LDS_BOYS_KT = 16
+
+
+def lds_boys_setup(L):
+    """Kernel-start copy of the coarse Boys rows of order L into LDS (every
+    thread of the workgroup, then a barrier); quartets then evaluate F_L with
+    vlxmi::boys_lds instead of gathering a table row per lane."""
+    assert L + 11 <= 15, "coarse rows hold F_0..F_15"
+    return ["    __shared__ double lboys[VLX_BOYS_CROWS * VLX_BOYS_CTERMS];",
+            "    vlxmi::boys_lds_fill<%d>(boys_func_table, lboys);" % L,
+            "    __syncthreads();"]
 # ket record (LDS, doubles): 0 screening bound, 1 eta, 2 1/eta, 3-5 Q, 6-8 QC,
 # 9-11 CD, 12 sqrt(eta), then the per-component values from KREC0
 KREC0 = 13
@@ -50,7 +62,7 @@ def _types_in(cls):
 # common pieces
 # --------------------------------------------------------------------------
 
-def quartet_prologue(L, ty_geo="double", omega=False, ind="    "):
+def quartet_prologue(L, ty_geo="double", omega=False, ind="    ", lds_boys=False):
     """Quartet prefactors.  Expects in scope: zeta, sq_zeta = sqrt(zeta), inv_S1,
     P_x.., PA_x.., AB_x.. (bra), eta, sq_eta = sqrt(eta), inv_eta, Q_x.., QC_x..,
     CD_x.. (ket), boys tables.  One Newton-refined 1/sqrt(zeta + eta) gives both
@@ -67,7 +79,10 @@ def quartet_prologue(L, ty_geo="double", omega=False, ind="    "):
         o.append("double d2 = 1.0;")
         o.append("if (omega != 0.0) { d2 = omega * omega / (rho + omega * omega); Tb *= d2; Lambda *= sqrt(d2); }")
     o.append("double Fb[%d];" % (L + 1))
-    o.append("vlxmi::boys<%d>(Tb, boys_func_table, Fb);" % L)
+    if lds_boys:
+        o.append("vlxmi::boys_lds<%d>(Tb, lboys, Fb);" % L)
+    else:
+        o.append("vlxmi::boys<%d>(Tb, boys_func_table, Fb);" % L)
     if omega and L > 0:
         o.append("if (omega != 0.0) { double dp = d2; "
                  + " ".join("Fb[%d] *= dp; dp *= d2;" % m for m in range(1, L + 1)) + " }")
@@ -338,7 +353,7 @@ def ket_record_bytes(kind, cls):
     return rec * 8 + (NC[d] * 4 if kind == "K" else 0)
 
 
-def emit_j(cls, mixed=False, batched=False, kt=None, layout="reg"):
+def emit_j(cls, mixed=False, batched=False, kt=None, layout="reg", lds_boys=False):
     """Source of computeCoulombFock{CLS} (or its _mixed / _batched variants) and its
     C-ABI launcher.  The batched variant is persistent: workgroup w processes
     work items w, w + gridDim.x, ...; item t = (bra block begin, bra block end
@@ -361,9 +376,11 @@ def emit_j(cls, mixed=False, batched=False, kt=None, layout="reg"):
     o.append("%s(%s)" % (name, ",\n    ".join(sig)))
     o.append("{")
     o.append("    constexpr int NA = %d, NB = %d, NC = %d, ND = %d, NAB = %d, NCD = %d;" % (na, nb, nc, nd, nab, ncd))
-    o.append("    constexpr int KT = %d, REC = %d;" % (kt or KET_TILE, rec))
+    o.append("    constexpr int KT = %d, REC = %d;" % (LDS_BOYS_KT if lds_boys else (kt or KET_TILE), rec))
     o.append("    __shared__ double ket[KT * REC];")
     o.append("    const uint32_t lane = threadIdx.x;")
+    if lds_boys:
+        o += lds_boys_setup(L)
     o.append("    const uint32_t nbra = %s_prim_pair_count_local / NAB;" % ab)
     o.append("    const uint32_t nket = %s_prim_pair_count / NCD;" % cd)
     if batched:
@@ -461,7 +478,7 @@ def emit_j(cls, mixed=False, batched=False, kt=None, layout="reg"):
     o.append("        const uint32_t nt = (k_end - t0 < (uint32_t)KT) ? (k_end - t0) : (uint32_t)KT;")
 
     def jbody(tyq, ind, groups):
-        lines = quartet_prologue(L, ty_geo=tyq, ind=ind)
+        lines = quartet_prologue(L, ty_geo=tyq, ind=ind, lds_boys=lds_boys)
         if jlds:
             lines += [ind + "uint32_t lid = lane;", ind + "asm volatile(\"\" : \"+v\"(lid));"]
         for isl, group in enumerate(groups):
@@ -541,7 +558,7 @@ def emit_j(cls, mixed=False, batched=False, kt=None, layout="reg"):
 # K kernel
 # --------------------------------------------------------------------------
 
-def emit_k(cls, mixed=False, batched=False, kt=None, layout="reg", dorder="jl"):
+def emit_k(cls, mixed=False, batched=False, kt=None, layout="reg", dorder="jl", lds_boys=False):
     """Source of computeExchangeFock{CLS} (or its _mixed / _batched variants) and
     its launcher.  The batched variant is persistent over a list of (i,k) blocks
     of lists that concatenate many molecules; per block, blk_mol gives the
@@ -576,7 +593,7 @@ def emit_k(cls, mixed=False, batched=False, kt=None, layout="reg", dorder="jl"):
     o.append("%s(%s)" % (name, ",\n    ".join(sig)))
     o.append("{")
     o.append("    constexpr int NA = %d, NB = %d, NC = %d, ND = %d, NAC = %d, NAB = %d, NCD = %d;" % (na, nb, nc, nd, nac, nab, ncd))
-    o.append("    constexpr int KT = %d, REC = %d;" % (kt or (32 if heavy else KET_TILE), rec))
+    o.append("    constexpr int KT = %d, REC = %d;" % (LDS_BOYS_KT if lds_boys else (kt or (32 if heavy else KET_TILE)), rec))
     o.append("    __shared__ double ket[KT * REC];")
     o.append("    __shared__ uint32_t ket_cg[KT * ND];")
     # heavy classes keep K accumulators in LDS when they fit beside the
@@ -598,6 +615,8 @@ def emit_k(cls, mixed=False, batched=False, kt=None, layout="reg", dorder="jl"):
         # own NA accumulators in registers for the duration of its ket loop
         o.append("    __shared__ double lacc[NAC * TILE_SIZE_K];")
     o.append("    const uint32_t lane = threadIdx.x;")
+    if lds_boys:
+        o += lds_boys_setup(L)
     if batched:
         o.append("    (void)%s_max_D;" % bd)
         o.append("    for (uint32_t w = blockIdx.x; w < nlist; w += gridDim.x) {")
@@ -744,7 +763,7 @@ def emit_k(cls, mixed=False, batched=False, kt=None, layout="reg", dorder="jl"):
         # the density gathers follow the Boys evaluation: vector loads complete
         # in order, so gathers issued before the Boys table loads only lengthen
         # the wait for the table and the live ranges
-        lines = quartet_prologue(L, ty_geo=tyq, omega=True, ind=ind)
+        lines = quartet_prologue(L, ty_geo=tyq, omega=True, ind=ind, lds_boys=lds_boys)
         TAU = (lambda t: "tau[%d * TILE_SIZE_K + lid]" % t) if heavy else (lambda t: "tau[%d]" % t)
         SAB = (lambda t: "Sab[%d * TILE_SIZE_K + lid]" % t) if heavy else (lambda t: "Sab[%d]" % t)
         if heavy:

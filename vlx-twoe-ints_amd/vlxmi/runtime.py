@@ -77,6 +77,7 @@ def lib():
     L.vlxmi_boys_table_size.argtypes = [C.POINTER(C.c_uint32)] * 3
     L.vlxmi_boys_table_fill.argtypes = [C.c_void_p, C.c_void_p]
     L.vlxmi_boys_eval.argtypes = [C.c_void_p, C.c_uint32, C.c_uint32, C.c_void_p, C.c_void_p, C.c_void_p]
+    L.vlxmi_boys_eval_coarse.argtypes = [C.c_void_p, C.c_uint32, C.c_uint32, C.c_void_p, C.c_void_p, C.c_void_p]
     L.vlxmi_status_string.restype = C.c_char_p
     L.vlxmi_status_string.argtypes = [C.c_int]
     L.vlxmi_version.restype = C.c_int
@@ -96,7 +97,8 @@ def check(rc: int, what: str = ""):
 def exported_symbols():
     """Names every C-ABI entry point the header declares."""
     names = ["vlxmi_version", "vlxmi_boys_table_size", "vlxmi_boys_table_fill", "vlxmi_status_string",
-             "vlxmi_boys_eval", "vlxmi_count_k_quartets", "vlxmi_segment_sum", "vlxmi_fill_zero"]
+             "vlxmi_boys_eval", "vlxmi_boys_eval_coarse", "vlxmi_count_k_quartets", "vlxmi_segment_sum",
+             "vlxmi_fill_zero"]
     names += ["vlxmi_launch_computeCoulombFock" + (ab + cd).upper() for ab, cd in j_classes()]
     names += ["vlxmi_launch_computeExchangeFock" + c.upper() for c in k_classes()]
     names += ["vlxmi_launch_computeCoulombFock" + (ab + cd).upper() + "_mixed" for ab, cd in j_classes()]
@@ -143,12 +145,14 @@ class Device:
             a = a.view(np.int32)
         return t.from_numpy(a).to(self.device, dtype=dtype)
 
-    def boys_eval(self, T: np.ndarray, L: int) -> np.ndarray:
+    def boys_eval(self, T: np.ndarray, L: int, coarse: bool = False) -> np.ndarray:
+        """F_0..F_L at T by the device Boys function (coarse: the LDS path)."""
         t = self.torch
         Td = self.tensor(np.asarray(T, dtype=np.float64))
         out = t.zeros(Td.numel() * (L + 1), dtype=t.float64, device=self.device)
-        check(lib().vlxmi_boys_eval(Td.data_ptr(), Td.numel(), L, self.boys_table.data_ptr(), out.data_ptr(),
-                                    self.stream_ptr()), "boys_eval")
+        fn = lib().vlxmi_boys_eval_coarse if coarse else lib().vlxmi_boys_eval
+        check(fn(Td.data_ptr(), Td.numel(), L, self.boys_table.data_ptr(), out.data_ptr(), self.stream_ptr()),
+              "boys_eval")
         return out.cpu().numpy().reshape(-1, L + 1)
 
     def schwarz(self, la, lb, info_a, info_b, I, J, S):

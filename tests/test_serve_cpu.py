@@ -61,3 +61,26 @@ def test_realtime_latency_accounting():
     reqs = [Request(rid=k, mol_id=k % 2, iteration=k // 2, arrival=0.002 * k, D=k) for k in range(6)]
     st = srv.serve(reqs, realtime=True)
     assert st.n == 6 and st.p99_ms >= st.p50_ms >= 0.0 and st.wall_s >= 0.01
+
+
+def test_time_breakdown_and_batch_order_rule():
+    """Setup, packing and build times are reported; a molecule whose earlier
+    iteration is still pending is held back even when it arrives first in the
+    queue order (requests given out of iteration order)."""
+    import time as _t
+
+    def slow_entry(mol):
+        _t.sleep(0.01)
+        return FakeEntry(mol)
+
+    mols = {m: {"bytes": 10} for m in range(2)}
+    srv = FockServer(mols, hbm_budget=1e9, max_batch=4, keep_results=True, entry_factory=slow_entry,
+                     batch_runner=_runner, synchronize=lambda: None)
+    reqs = [Request(rid=0, mol_id=0, iteration=1, arrival=0.0, D=0),
+            Request(rid=1, mol_id=0, iteration=0, arrival=0.0, D=1),
+            Request(rid=2, mol_id=1, iteration=0, arrival=0.0, D=2)]
+    st = srv.serve(reqs, realtime=False)
+    assert st.setups == 2 and st.setup_s >= 0.02 and st.pack_s == 0.0 and st.build_s >= 0.0
+    first = srv.batch_log[0]
+    assert 0 not in first and 1 in first and 2 in first        # iteration 1 waits for iteration 0
+    assert srv.batch_log[1] == [0]

@@ -58,7 +58,9 @@ def oracle_matJ(fl, D_cart, mode="component", bra_blocks=None, classes=None, n_e
 
 
 def oracle_matK(fl, D_cart, mode="component", ik_blocks=None, classes=None, omega=0.0, max_j=0, n_eval=None,
-                timer=None):
+                timer=None, max_D=None):
+    """Per (a,c) class mat_K by the oracle; ``max_D`` overrides the screening's
+    maxD_bd (by default the largest |D| over the class's (b,d) AO block)."""
     out = {}
     infos = [t.info if t is not None else None for t in fl.prims]
     aos = [t.aoinds if t is not None else None for t in fl.prims]
@@ -70,7 +72,7 @@ def oracle_matK(fl, D_cart, mode="component", ik_blocks=None, classes=None, omeg
             continue
         ik = fl.iklists[ac]
         la, lb, lc, ld = P.LVAL[a], P.LVAL[b], P.LVAL[c], P.LVAL[d]
-        mx = P.max_abs_D_block(D_cart, fl.prims, lb, ld)
+        mx = P.max_abs_D_block(D_cart, fl.prims, lb, ld) if max_D is None else float(max_D)
         if ac not in out:
             out[ac] = np.zeros(ik.i.size)
         A, Cc = fl.kcsr[ab], fl.kcsr[cd]

@@ -106,3 +106,31 @@ def test_fock_small8(small8):
     assert np.max(np.abs(K - Kb)) < 1e-9
     EJ, EK = eng.energies(D, eng.dev.tensor(J), eng.dev.tensor(K))
     assert abs(EJ - 0.5 * np.sum(D * Jr)) < 1e-9 and EJ > 0 and EK > 0
+
+
+def test_k_density_order_contract(small8):
+    """Exchange classes listed in k_dorder.json read D_jl as D[l][j]: with a
+    non-symmetric density they give the oracle's K for D^T, the other classes
+    the oracle's K for D (the engines symmetrise the density, so both agree
+    there; this pins what a caller passing a non-symmetric D gets)."""
+    from vlxmi.gen import build as B
+    mol, D, eng = small8
+    Dc = U.cart_density(mol, D)
+    rng = np.random.default_rng(5)
+    A = rng.normal(size=Dc.shape) * 1e-3
+    Dn = Dc + (A - A.T)
+    Dt = eng.dev.tensor(np.ascontiguousarray(Dn))
+    lj = B.load_layouts(B.DORDER_TABLE)
+    kinds = set()
+    mx = float(np.abs(Dn).max())
+    for cls in ("spss", "spsp", "sppp", "pppp", "pspp"):
+        a, b, c, d = cls
+        _say("case: small8 K class %s, non-symmetric density (%s)" % (cls, "D[l][j]" if cls in lj else "D[j][l]"))
+        g = eng.run_K(Dt, classes=[cls], maxD={b + d: mx})[a + c].cpu().numpy()
+        read, other = (Dn.T, Dn) if cls in lj else (Dn, Dn.T)
+        r = U.oracle_matK(eng.fl, np.ascontiguousarray(read), "component", classes=[cls], max_D=mx)[a + c]
+        r2 = U.oracle_matK(eng.fl, np.ascontiguousarray(other), "component", classes=[cls], max_D=mx)[a + c]
+        assert np.max(np.abs(g - r)) <= TOL, (cls, np.max(np.abs(g - r)))
+        assert np.max(np.abs(g - r2)) > 1e3 * TOL, cls          # the read order is observable
+        kinds.add(cls in lj)
+    assert kinds == {True, False}

@@ -243,7 +243,7 @@ def main():
     a = ap.parse_args()
     if a.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py: this build runs the J/K path on one GPU only; --gpus %d is not supported "
-              "(the multi-GPU sharding is described in DESIGN.md, not built)." % a.gpus, file=sys.stderr)
+              "(sharding across GPUs is not built)." % a.gpus, file=sys.stderr)
         sys.exit(2)
 
     if a.mode == "batch":

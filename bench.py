@@ -242,7 +242,7 @@ def main():
                     help="directory with FETCH_SIZE/WRITE_SIZE counter runs of this command (fills roofline.traffic)")
     a = ap.parse_args()
     if a.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
-        print("bench.py: this build runs the J/K path on one GPU only; --gpus %d is not supported "
+        print("bench.py: the J/K path runs on one GPU only; --gpus %d is not supported "
               "(sharding across GPUs is not built)." % a.gpus, file=sys.stderr)
         sys.exit(2)
 

@@ -1,7 +1,7 @@
 /*
  * FP64 CPU oracle for the Obara-Saika J/K Fock path.
  *
- * TEST INFRASTRUCTURE ONLY.  This file is the parity checker: only tests/,
+ * TEST INFRASTRUCTURE ONLY.  This file is the parity reference: only tests/,
  * __graft_entry__.smoke() and bench.py's cpu_baseline leg may load it.  The
  * product path (vlx-twoe-ints_amd/) never links or calls it.
  *

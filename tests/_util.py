@@ -1,4 +1,4 @@
-"""Shared helpers for the test suite (checker side).
+"""Shared helpers for the test suite (oracle side).
 
 ``oracle_fock`` runs the CPU oracle over the same reference-layout lists the HIP
 kernels consume and assembles AO J/K with the product's host assembly code.

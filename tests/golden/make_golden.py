@@ -7,7 +7,7 @@
                    primitive quartet per class (s,p,d)^4, with their inputs.
 
 The reference repository holds no numeric fixtures; these are produced by the
-checkers of this repository (the MD evaluator is independent of the oracle).
+independent evaluators of this repository (the MD evaluator does not use the oracle).
 """
 import math
 import os

@@ -215,7 +215,7 @@ def _compile(src_path, obj_path, extra=()):
 
 def kernel_resources(obj_path):
     """{kernel name: (registers per lane, LDS bytes)} of the gfx950 code object
-    inside one of this build's host objects."""
+    inside a host object compiled by this module."""
     with tempfile.TemporaryDirectory() as td:
         fb, dv = os.path.join(td, "fb.bin"), os.path.join(td, "dev.o")
         subprocess.run([LLVM + "/llvm-objcopy", "--dump-section", ".hip_fatbin=" + fb, obj_path], check=True)

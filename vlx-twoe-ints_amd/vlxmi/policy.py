@@ -1,7 +1,7 @@
 """Which classes run their ``_mixed`` kernel in mixed precision.
 
 gfx950 issues FP32 VALU instructions at the FP64 rate when packed FP32 is not
-used (this build compiles without it), so running a class's VRR/HRR in FP32
+used (the library is compiled without it), so running a class's VRR/HRR in FP32
 only pays where halving the recursion's registers removes spills; in the light
 classes the conversions make the ``_mixed`` kernel slower than the FP64 one.
 ``gen/mixed_classes.json`` lists the classes whose ``_mixed`` kernel measured

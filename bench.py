@@ -1,15 +1,22 @@
 #!/usr/bin/env python
 """J+K Fock-build throughput on one MI355X.
 
-One step = one full J+K build (all 36 Coulomb and 54 exchange classes, the
-device list preparation for the density, and AO assembly) of the lattice64
-workload: 64 atoms on a 4x4x4 lattice (2.5 bohr, jitter 0.3), per atom 3 s
-(3 prims), 2 p (2 prims), 1 spherical d (1 prim), nbf 896 (960 Cartesian),
-D = C C^T with C (896 x 224) ~ N(0, 0.1^2), Schwarz threshold 1e-12.  Inputs
-(pair lists, Schwarz factors, density) are resident in HBM before timing.
+Default (``--mode batch``, the headline): one step = the J+K builds of 512
+molecules (atoms floor(log-uniform[4, 48]) uniform in [-3, 3]^3 bohr at 1.5 bohr
+minimum separation, per atom 2 s (3 prims) + 1 p (2 prims), D = C C^T), all
+quartet lists of one angular class in one persistent launch under a captured
+HIP graph, in mixed precision (FP32 VRR/HRR below the Schwarz x |D| bound) and
+in FP64; ``value`` is the mixed batched rate, reported beside the batched FP64
+rate and the plain FP64 single-molecule path (the per-class launches, one
+molecule at a time) on the same molecules, with the energy errors.  Inputs
+(pair lists, Schwarz factors, densities) are resident in HBM before timing.
 
-Prints one JSON line (see the keys below).  ``--gpus N`` with N > 1 exits: this
-build runs on one GPU.
+``--mode fock --workload lattice64|lattice256|...``: one molecule's full J+K
+build (all 36 Coulomb and 54 exchange classes) FP64 and mixed.
+``--mode serve``: the request server over Poisson arrivals.
+
+Prints one JSON line.  ``--gpus N`` with N > 1 exits: this build runs on one
+GPU.
 """
 
 import argparse
@@ -60,7 +67,7 @@ def cpu_baseline(eng, D_cart, nq, seed=0):
     from vlxmi import pairs as P
     fl = eng.fl
     rng = np.random.default_rng(seed)
-    cores = int(os.environ.get("OMP_NUM_THREADS", "0")) or min(16, os.cpu_count() or 1)
+    cores = oracle_threads()
     os.environ["OMP_NUM_THREADS"] = str(cores)
     est = spent = 0.0
     nsamp = 0
@@ -92,21 +99,123 @@ def cpu_baseline(eng, D_cart, nq, seed=0):
         nsamp += int(ne[0])
         if ne[0]:
             est += dt / float(ne[0]) * nq[name][0]
+    info = host_cpu_info()
     return {"value": 1.0 / est, "unit": "builds/s", "cores": cores, "kind": "port",
-            "sample": "FP64 C oracle (OpenMP, %d threads): per class, %d random bra blocks (J) or (i,k) blocks "
-                      "capped at 8 partner blocks (K); %d primitive quartets evaluated in %.1f s, seconds per "
-                      "quartet x the class's surviving-quartet count -> %.1f s per full J+K build"
-                      % (cores, cores, nsamp, spent, est),
-            "seconds_per_build": est}
+            "sample": "FP64 C oracle (OpenMP, %d threads; host %s, nproc %d, %d CPUs available to this process): "
+                      "per class, %d random bra blocks (J) or (i,k) blocks capped at 8 partner blocks (K); "
+                      "%d primitive quartets evaluated in %.1f s, seconds per quartet x the class's "
+                      "surviving-quartet count -> %.1f s per full J+K build"
+                      % (cores, info["cpu_model"], info["nproc"], info["cpus_available"], cores, nsamp, spent, est),
+            "seconds_per_build": est, **info}
+
+
+def host_cpu_info():
+    """Host CPU facts recorded beside the CPU baseline."""
+    model = "unknown"
+    try:
+        with open("/proc/cpuinfo") as f:
+            for ln in f:
+                if ln.startswith("model name"):
+                    model = ln.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    try:
+        avail = len(os.sched_getaffinity(0))
+    except AttributeError:
+        avail = os.cpu_count() or 1
+    return {"cpu_model": model, "nproc": os.cpu_count() or 1, "cpus_available": avail}
+
+
+def oracle_threads():
+    """Threads the CPU oracle runs on: the OMP_NUM_THREADS this host sets for
+    the process (the host's CPU share for this GPU), else every available CPU."""
+    info = host_cpu_info()
+    return int(os.environ.get("OMP_NUM_THREADS", "0")) or info["cpus_available"]
+
+
+def cpu_baseline_batch(entries, dens, counts, nmol, seed=0, budget_s=20.0):
+    """CPU oracle seconds per batch, estimated from a bounded sample.
+
+    Per class: bra blocks (J) or (i,k) blocks (K) of randomly chosen molecules
+    of the batch are evaluated by the oracle (FP64 C restatement of the
+    recursions, OpenMP); seconds per evaluated quartet x the class's quartet
+    count over the whole batch (the batched kernels' counters) gives the
+    class's CPU time.  The sample of every class is sized to about
+    ``budget_s`` / (number of classes) seconds."""
+    import numpy as np
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import _util as U
+    from vlxmi import pairs as P
+    rng = np.random.default_rng(seed)
+    cores = oracle_threads()
+    os.environ["OMP_NUM_THREADS"] = str(cores)
+    dcache = {}
+
+    def dcart(m):
+        if m not in dcache:
+            dcache[m] = U.cart_density(entries[m].mol, dens[m])
+        return dcache[m]
+
+    est = spent = 0.0
+    nsamp = 0
+    jobs = [("J", ab + cd) for ab, cd in P.j_classes()] + [("K", c) for c in P.k_classes()]
+    per_class = budget_s / len(jobs)
+    for kind, cls in jobs:
+        nq = sum(counts.get("%s:%s" % (kind, cls), (0, 0)))
+        if nq == 0:
+            continue
+        t_cls, ne_cls = 0.0, 0
+        tries = 0
+        while t_cls < per_class and tries < 64:
+            tries += 1
+            m = int(rng.integers(nmol))
+            fl = entries[m].fl
+            ne = np.zeros(1, dtype=np.uint64)
+            tm = []
+            if kind == "J":
+                ab, cd = cls[:2], cls[2:]
+                if ab not in fl.jlists or cd not in fl.jlists:
+                    continue
+                n = fl.jlists[ab].nblk
+                sel = np.sort(rng.choice(n, size=min(n, cores), replace=False)).astype(np.uint32)
+                U.oracle_matJ(fl, dcart(m), "component", bra_blocks=sel, classes=[(ab, cd)], n_eval=ne, timer=tm)
+            else:
+                ac = cls[0] + cls[2]
+                if ac not in fl.iklists or fl.iklists[ac].nblk == 0:
+                    continue
+                n = fl.iklists[ac].nblk
+                sel = np.sort(rng.choice(n, size=min(n, cores), replace=False)).astype(np.uint32)
+                U.oracle_matK(fl, dcart(m), "component", ik_blocks=sel, classes=[cls], n_eval=ne, timer=tm)
+            t_cls += sum(tm)
+            ne_cls += int(ne[0])
+        spent += t_cls
+        nsamp += ne_cls
+        if ne_cls:
+            est += t_cls / ne_cls * nq
+    info = host_cpu_info()
+    return {"value": nmol / est, "unit": "builds/s", "cores": cores, "kind": "port",
+            "sample": "FP64 C oracle (OpenMP, %d threads; host %s, nproc %d, %d CPUs available to this process): "
+                      "per class, blocks of random molecules of the batch (%d primitive quartets in %.1f s), "
+                      "seconds per quartet x the class's quartet count over the batch -> %.1f s per batch of %d "
+                      "molecules" % (cores, info["cpu_model"], info["nproc"], info["cpus_available"], nsamp, spent,
+                                     est, nmol),
+            "seconds_per_batch": est, **info}
+
+
+def _energy(dev, D, J):
+    return 0.5 * float((dev.tensor(D) * J).sum())
 
 
 def bench_batch(a):
-    """Many small molecules (4..48 atoms, 2s1p basis per atom) in one persistent
-    launch per class under a captured HIP graph, FP64 and mixed; compared with
-    the single-molecule path on the first molecules."""
+    """Headline: many molecules (config-4 generator: atoms floor(log-uniform
+    [4, 48]) in [-3, 3]^3 bohr at 1.5 bohr minimum separation, 2 s (3 prims) +
+    1 p (2 prims) per atom, D = C C^T) built together, one persistent launch per
+    class under a captured HIP graph, FP64 and mixed; against the plain FP64
+    single-molecule path on the same molecules."""
     import numpy as np
     import torch
-    from vlxmi import configs
+    from vlxmi import configs, flops
     from vlxmi.batch import BatchEngine, MolEntry, Pack
     from vlxmi.fock import FockEngine
     from vlxmi.runtime import Device
@@ -122,16 +231,12 @@ def bench_batch(a):
         if (i + 1) % 64 == 0:
             _progress("batch: %d molecule entries resident" % (i + 1))
     pack = Pack(entries, dev)
-    _progress("batch: pack built")
     dens = [D for _, D in mols]
     torch.cuda.synchronize()
     setup = time.time() - t0
-    out = {"metric": "batched J+K builds/s (molecules per second, one persistent launch per class, HIP graph)",
-           "unit": "builds/s", "n_gpus": 1, "steps": a.steps, "warmup": a.warmup, "higher_is_better": True,
-           "data": "synthetic", "config": {"workload": "batch: %d molecules, floor(log-uniform[4,%d]) atoms, "
-                                                       "2 s (3 prims) + 1 p (2 prims) per atom" % (nmol, hi),
-                                           "atoms_total": int(sum(m.natoms for m, _ in mols))},
-           "setup_s": setup}
+    _progress("batch: pack built (%.1f s of set-up)" % setup)
+    model = flops.load_model()
+    res = {}
     for prec in ("fp64", "mixed"):
         be = BatchEngine(dev, precision=prec, fp32_threshold=a.fp32_threshold, mixed_policy=a.mixed_policy)
         tune = be.autotune(pack, dens, reps=1, progress=_progress)
@@ -141,38 +246,151 @@ def bench_batch(a):
         for _ in range(a.warmup):
             replay(dens)
         torch.cuda.synchronize()
-        t0 = time.perf_counter()
+        t1 = time.perf_counter()
         for _ in range(a.steps):
             J, K = replay(dens)
         torch.cuda.synchronize()
-        dt = time.perf_counter() - t0
-        out[prec] = {"builds_per_s": nmol * a.steps / dt, "ms_per_batch": 1e3 * dt / a.steps,
-                     "autotuned_classes": len(tune)}
-        if prec == "fp64":
-            J64, K64 = J, K
-        else:
-            dEJ = max(abs(0.5 * float((dev.tensor(D) * (Jm - Jf)).sum())) for D, Jm, Jf in zip(dens, J, J64))
-            dEK = max(abs(0.5 * float((dev.tensor(D) * (Km - Kf)).sum())) for D, Km, Kf in zip(dens, K, K64))
-            out[prec]["max_abs_dE_J"] = dEJ
-            out[prec]["max_abs_dE_K"] = dEK
-        _progress("batch %s: %.2f builds/s" % (prec, out[prec]["builds_per_s"]))
-    # single-molecule path (eager, one molecule at a time) on the first molecules, scaled
-    nsub = min(16, nmol)
-    engs = [FockEngine(mols[m][0], dev=dev) for m in range(nsub)]
-    for m in range(nsub):
+        dt = time.perf_counter() - t1
+        _progress("batch %s: %.2f builds/s" % (prec, nmol * a.steps / dt))
+        # per-launch times (HIP events on the launch stream) and quartet counts of
+        # the same build, launched eagerly
+        be.prepare(pack, dens)
+        per = {}
+        for _ in range(2):
+            be.timer = []
+            be.launch(pack)
+            torch.cuda.synchronize()
+            for name, e0, e1 in be.timer:
+                per.setdefault(name, []).append(e0.elapsed_time(e1) * 1e-3)
+        be.timer = None
+        counts = be.quartet_counts(pack)
+        res[prec] = dict(dt=dt, J=J, K=K, per=per, counts=counts, tuned=len(tune))
+    # plain FP64 path, one molecule at a time (the drop-in per-class launches)
+    engs = [FockEngine(mol, dev=dev, lists=e.fl) for (mol, _), e in zip(mols, entries)]
+    for m in range(nmol):
         engs[m].build(dens[m])
     torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    for m in range(nsub):
-        engs[m].build(dens[m])
+    _progress("batch: single-molecule engines ready")
+    t1 = time.perf_counter()
+    single = [engs[m].build(dens[m]) for m in range(nmol)]
     torch.cuda.synchronize()
-    t1 = (time.perf_counter() - t0) / nsub
-    _progress("batch: single-molecule path timed")
-    out["single_molecule_path"] = {"molecules_timed": nsub, "s_per_molecule": t1}
-    out["value"] = out["mixed"]["builds_per_s"]
-    out["speedup_batched_fp64_vs_single"] = out["fp64"]["builds_per_s"] * t1
-    out["mixed_vs_fp64"] = out["mixed"]["builds_per_s"] / out["fp64"]["builds_per_s"]
+    dt_single = time.perf_counter() - t1
+    _progress("batch: single-molecule FP64 path %.2f builds/s" % (nmol / dt_single))
+
+    # errors
+    def dmax(X, Y):
+        return max(float((x - y).abs().max()) for x, y in zip(X, Y))
+
+    EJs = [_energy(dev, D, J) for D, (J, _) in zip(dens, single)]
+    EKs = [_energy(dev, D, K) for D, (_, K) in zip(dens, single)]
+    err = {}
+    for prec in ("fp64", "mixed"):
+        r = res[prec]
+        dEJ = [_energy(dev, D, J) - e for D, J, e in zip(dens, r["J"], EJs)]
+        dEK = [_energy(dev, D, K) - e for D, K, e in zip(dens, r["K"], EKs)]
+        err[prec] = {"max_abs_dE_J": max(abs(x) for x in dEJ), "max_abs_dE_K": max(abs(x) for x in dEK),
+                     "max_abs_dJ": dmax(r["J"], [s[0] for s in single]),
+                     "max_abs_dK": dmax(r["K"], [s[1] for s in single])}
+    n32 = sum(v[1] for v in res["mixed"]["counts"].values())
+    nall = sum(v[0] + v[1] for v in res["mixed"]["counts"].values())
+
+    # roofline of the dominant kernel of the mixed batched build
+    def flop_of(name, counts):
+        kind = "J" if name.startswith("computeCoulomb") else "K"
+        cls = name.replace("computeCoulombFock", "").replace("computeExchangeFock", "").split("_")[0].lower()
+        nq = sum(counts.get("%s:%s" % (kind, cls), (0, 0)))
+        return nq, nq * model[kind][cls]["total"]
+
+    per = res["mixed"]["per"]
+    dom = max(per, key=lambda k: sum(per[k]))
+    dom_t = sum(per[dom]) / len(per[dom])
+    dom_q, dom_f = flop_of(dom, res["mixed"]["counts"])
+    tot_f = sum(flop_of(k, res["mixed"]["counts"])[1] for k in per)
+    kern_s = sum(sum(v) / len(v) for v in per.values())
+    traffic, traffic_src = None, None
+    tab = os.path.join(ROOT, "profiles", "hbm_traffic_batch.json")
+    if os.path.exists(tab):
+        with open(tab) as f:
+            tk = json.load(f)["kernels"].get(dom)
+        if tk:
+            traffic = tk["read"] + tk["write"]
+            traffic_src = "profiles/hbm_traffic_batch.json (FETCH_SIZE/WRITE_SIZE counter runs, copy-calibrated)"
+    bps = {p: nmol * a.steps / res[p]["dt"] for p in res}
+    single_bps = nmol / dt_single
+    out = {
+        "metric": "J+K Fock builds/s, mixed batched (FP32 VRR/HRR below Schwarz x |D| %.0e) vs the plain FP64 "
+                  "single-molecule path, at <= 1e-8 Eh" % a.fp32_threshold,
+        "value": bps["mixed"], "unit": "builds/s", "n_gpus": 1, "steps": a.steps, "warmup": a.warmup,
+        "ms_per_step": 1e3 * res["mixed"]["dt"] / a.steps, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f64+f32", "data": "synthetic",
+        "config": {"workload": "batch: %d molecules, atoms floor(log-uniform[4,%d]) uniform in [-3,3]^3 bohr "
+                               "(min separation 1.5), 2 s (3 prims) + 1 p (2 prims) per atom, D = C C^T, "
+                               "C (nbf x ceil(nbf/4)) ~ N(0, 0.3^2), thr 1e-12; one step = all %d molecules "
+                               "(one persistent launch per class, HIP graph)" % (nmol, hi, nmol),
+                   "atoms_total": int(sum(m.natoms for m, _ in mols)), "molecules": nmol,
+                   "precision": "mixed (fp32 threshold %.0e, %s classes)" % (a.fp32_threshold, a.mixed_policy)},
+        "roofline": {"bound": "valu", "kernel": dom, "achieved": dom_f / dom_t / 1e12, "peak": PEAK_FP64_VALU / 1e12,
+                     "unit": "TFLOP/s", "frac": dom_f / dom_t / PEAK_FP64_VALU, "traffic": traffic,
+                     "traffic_source": traffic_src, "algorithmic_flop_per_launch": dom_f,
+                     "quartets_per_launch": dom_q, "avg_launch_s": dom_t},
+        "cpu_baseline": None,
+        "batched_mixed_builds_per_s": bps["mixed"],
+        "batched_fp64_builds_per_s": bps["fp64"],
+        "single_fp64_builds_per_s": single_bps,
+        "mixed_batched_vs_single_fp64": bps["mixed"] / single_bps,
+        "mixed_batched_vs_batched_fp64": bps["mixed"] / bps["fp64"],
+        "batched_fp64_vs_single_fp64": bps["fp64"] / single_bps,
+        "errors_vs_single_fp64": err,
+        "fp32_quartet_fraction": n32 / max(nall, 1),
+        "whole_batch": {"algorithmic_flop": tot_f, "kernel_s": kern_s, "tflops_over_kernels": tot_f / kern_s / 1e12},
+        "top_kernels_ms": {k: 1e3 * sum(per[k]) / len(per[k]) for k in sorted(per, key=lambda k: -sum(per[k]))[:6]},
+        "autotuned_classes": {p: res[p]["tuned"] for p in res}, "setup_s": setup,
+    }
+    if not a.no_cpu:
+        _progress("batch: CPU oracle baseline (bounded sample)")
+        out["cpu_baseline"] = cpu_baseline_batch(entries, dens, res["mixed"]["counts"], nmol)
+    if not a.no_secondary:
+        out["secondary_lattice64"] = lattice64_secondary(a, dev)
     print(json.dumps(out), flush=True)
+
+
+def lattice64_secondary(a, dev):
+    """The plain FP64 lattice64 build (nbf 896, all classes through (dd|dd)):
+    builds/s on the GPU beside the CPU oracle's rate on a bounded sample."""
+    import torch
+    from vlxmi import configs, flops
+    from vlxmi import pairs as P
+    from vlxmi.fock import FockEngine
+    mol, D = configs.make_lattice64()
+    eng = FockEngine(mol, dev=dev)
+    Dd = dev.tensor(D)
+    eng.build(Dd)
+    torch.cuda.synchronize()
+    steps = 2
+    t1 = time.perf_counter()
+    for _ in range(steps):
+        eng.build(Dd)
+    torch.cuda.synchronize()
+    dt = (time.perf_counter() - t1) / steps
+    _progress("lattice64: FP64 build %.3f s" % dt)
+    out = {"workload": WORKLOAD_TEXT["lattice64"] % {"nbf": mol.nbf, "ncart": mol.naos_cart, "nocc": mol.nbf // 4},
+           "fp64_builds_per_s": 1.0 / dt, "fp64_build_s": dt}
+    if not a.no_cpu:
+        model = flops.load_model()
+        Dc = eng.cart_density(Dd)
+        Dc_np = Dc.cpu().numpy()
+        maxD = eng.max_D(Dc)
+        W = {}
+        for ab, cd in P.j_classes():
+            if ab in eng.fl.jlists and cd in eng.fl.jlists:
+                nq = flops.count_j(eng.fl, Dc_np, ab, cd)
+                W["computeCoulombFock" + (ab + cd).upper()] = (nq, nq * model["J"][ab + cd]["total"])
+        for cls in P.k_classes():
+            nq = eng.count_k_quartets(cls, maxD.get(cls[1] + cls[3], 0.0))
+            W["computeExchangeFock" + cls.upper()] = (nq, nq * model["K"][cls]["total"])
+        _progress("lattice64: CPU oracle baseline (bounded sample)")
+        out["cpu_baseline"] = cpu_baseline(eng, Dc_np, W)
+    return out
 
 
 def bench_serve(a):
@@ -232,8 +450,10 @@ def main():
     ap.add_argument("--mixed-policy", choices=("measured", "all"), default="measured",
                     help="mixed precision: launch _mixed kernels only for the classes of "
                          "vlxmi/gen/mixed_classes.json (measured faster) or for every class")
-    ap.add_argument("--mode", choices=("fock", "batch", "serve"), default="fock",
-                    help="fock: lattice64 J+K build (headline); batch: many molecules per launch; serve: request server")
+    ap.add_argument("--mode", choices=("fock", "batch", "serve"), default="batch",
+                    help="batch (default, the headline): 512 molecules per persistent launch vs the single-molecule "
+                         "FP64 path; fock: one molecule's J+K build (--workload); serve: request server")
+    ap.add_argument("--no-secondary", action="store_true", help="batch: skip the lattice64 FP64 / CPU figures")
     ap.add_argument("--molecules", type=int, default=None, help="batch/serve: number of molecules")
     ap.add_argument("--max-atoms", type=int, default=None, help="batch/serve: largest molecule (atoms)")
     ap.add_argument("--rates", default="5,20,80", help="serve: arrival rates (requests/s) to sweep")

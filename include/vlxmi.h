@@ -4,6 +4,7 @@
  * vlxmi status (vlxmi_status.h) for argument errors, or a positive hipError_t. */
 #ifndef VLXMI_H
 #define VLXMI_H
+#include <stddef.h>
 #include <stdint.h>
 #include "vlxmi_kernels.h"
 #ifdef __cplusplus
@@ -47,6 +48,23 @@ int vlxmi_segment_sum(const double* values, const uint32_t* perm, const uint32_t
 
 /* p[0..n) = 0 by a kernel (safe inside HIP-graph capture). */
 int vlxmi_fill_zero(double* p, uint64_t n, void* stream);
+
+/* Ket-split partial sums of the Coulomb launchers (no counterpart in the
+ * reference, whose J kernels give each bra entry one writer,
+ * gen_coulomb.py:285-297).  A Coulomb launch with ket_split > 1 stores chunk y's
+ * partial sums at part[y * count_local + e] in a workspace the library keeps per
+ * (current device, stream) -- launches on one stream share it, other streams get
+ * their own -- and then calls vlxmi_ket_split_reduce on the same stream.
+ * vlxmi_ket_workspace returns (in *out) a buffer of at least ndoubles doubles,
+ * growing it after synchronizing the stream; vlxmi_release_workspaces frees all
+ * of them after a device synchronization. */
+int vlxmi_ket_workspace(size_t ndoubles, void* stream, double** out);
+int vlxmi_release_workspaces(void);
+
+/* out[e] += sum_{y < nsplit} part[y * count + e] for e < count, the chunks added
+ * in order with compensated (Neumaier) summation: deterministic, about one
+ * rounding of the total. */
+int vlxmi_ket_split_reduce(const double* part, uint32_t nsplit, uint32_t count, double* out, void* stream);
 
 #ifdef __cplusplus
 }

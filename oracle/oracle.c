@@ -344,8 +344,10 @@ int orc_coulomb_class(int la, int lb, int lc, int ld,
             prim_of(prim_info[lb], prim_count[lb], J, &aj, B);
             double qmax = 0.0;
             for (int t = 0; t < nab; ++t) { double q = fabs(ab_mat_Q[e0 + t]); if (q > qmax) qmax = q; }
-            double acc[36];
-            for (int t = 0; t < nab; ++t) acc[t] = 0.0;
+            /* extended-precision sums: the oracle's own rounding stays far below
+               the parity bar even for sums of millions of quartets */
+            long double acc[36];
+            for (int t = 0; t < nab; ++t) acc[t] = 0.0L;
             for (uint32_t kb = 0; kb < nket; ++kb) {
                 if (mode == 1) {
                     if (!(qmax * beta[kb] > thr)) continue;
@@ -362,7 +364,7 @@ int orc_coulomb_class(int la, int lb, int lc, int ld,
                 for (int tab = 0; tab < nab; ++tab) {
                     const double Qij = ab_mat_Q[e0 + tab];
                     const double Sij = ab_pair_data[e0 + tab];
-                    double s_acc = 0.0;
+                    long double s_acc = 0.0L;
                     for (int tcd = 0; tcd < ncd; ++tcd) {
                         const size_t kl = f0 + tcd;
                         const double Dkl = cd_mat_D[kl];
@@ -370,12 +372,12 @@ int orc_coulomb_class(int la, int lb, int lc, int ld,
                         double w;
                         if (!same_cd) w = 2.0;
                         else w = (cd_first[kl] != cd_second[kl]) ? 2.0 : 1.0;
-                        s_acc += blk[tab * ncd + tcd] * Sij * cd_pair_data[kl] * Dkl * w;
+                        s_acc += (long double)(blk[tab * ncd + tcd] * Sij * cd_pair_data[kl] * Dkl * w);
                     }
                     acc[tab] += s_acc;
                 }
             }
-            for (int t = 0; t < nab; ++t) mat_J[e0 + t] += acc[t];
+            for (int t = 0; t < nab; ++t) mat_J[e0 + t] = (double)((long double)mat_J[e0 + t] + acc[t]);
         }
         free(blk);
     }
@@ -430,8 +432,8 @@ int orc_exchange_class(int la, int lb, int lc, int ld,
             uint32_t nJ = cnt_i / nb;
             const uint32_t nL = cnt_k / nd;
             if (max_j && nJ > max_j) nJ = max_j;
-            double acc[36];
-            for (int t = 0; t < nac; ++t) acc[t] = 0.0;
+            long double acc[36];
+            for (int t = 0; t < nac; ++t) acc[t] = 0.0L;
             for (uint32_t jb = 0; jb < nJ; ++jb) {
                 const uint32_t J = Dinds_ab[displs_ab[I * na] + jb * nb] / nb;
                 double aj, B[3];
@@ -461,7 +463,7 @@ int orc_exchange_class(int la, int lb, int lc, int ld,
                     ++nev;
                     for (int ca = 0; ca < na; ++ca)
                         for (int cc = 0; cc < nc; ++cc) {
-                            double s_acc = 0.0;
+                            long double s_acc = 0.0L;
                             for (int cb = 0; cb < nb; ++cb) {
                                 const size_t pj = displs_ab[I * na + ca] + jb * nb + cb;
                                 const uint32_t jprim = Dinds_ab[pj];
@@ -472,15 +474,15 @@ int orc_exchange_class(int la, int lb, int lc, int ld,
                                     const uint32_t lprim = Dinds_cd[pl];
                                     const uint32_t lcg = prim_aoinds[ld][(lprim / nd) + prim_count[ld] * (lprim % nd)];
                                     if (mode == 0 && !(fabs(Qij * Q_cd[pl] * max_D_bd) > thr)) continue;
-                                    s_acc += blk[((ca * nb + cb) * nc + cc) * nd + cd] * Sij * pd_cd[pl]
-                                             * D_AO[(size_t)jcg * naos + lcg];
+                                    s_acc += (long double)(blk[((ca * nb + cb) * nc + cc) * nd + cd] * Sij * pd_cd[pl]
+                                                           * D_AO[(size_t)jcg * naos + lcg]);
                                 }
                             }
                             acc[ca * nc + cc] += s_acc;
                         }
                 }
             }
-            for (int t = 0; t < nac; ++t) mat_K[e0 + t] += acc[t];
+            for (int t = 0; t < nac; ++t) mat_K[e0 + t] = (double)((long double)mat_K[e0 + t] + acc[t]);
         }
         free(blk);
     }

@@ -121,3 +121,28 @@ def test_batched_mixed_error(dev, mols, pack, policy):
         dEJ = 0.5 * float((Dt * (Jm[m] - J64[m])).sum().item())
         dEK = 0.5 * float((Dt * (Km[m] - K64[m])).sum().item())
         assert abs(dEJ) <= 1e-8 and abs(dEK) <= 1e-8, (m, dEJ, dEK)
+
+
+def test_quartet_counters_match_host_counts(dev, mols, pack):
+    """The batched kernels count the primitive quartets they evaluate; the count
+    must be the N_q of the FLOP model (quartets passing the block screen),
+    counted independently on the host from the same lists and densities."""
+    from vlxmi import flops
+    from vlxmi import pairs as P
+    from vlxmi.batch import BatchEngine
+    entries, pk = pack
+    _say("case: batched quartet counters vs host N_q")
+    be = BatchEngine(dev)
+    be.build(pk, [D for _, D in mols])
+    got = be.quartet_counts(pk)
+    want = {}
+    for e, (mol, D) in zip(entries, mols):
+        Dc = U.cart_density(mol, D)
+        for ab, cd in P.j_classes():
+            if ab in e.fl.jlists and cd in e.fl.jlists:
+                want["J:" + ab + cd] = want.get("J:" + ab + cd, 0) + flops.count_j(e.fl, Dc, ab, cd)
+        for cls in P.k_classes():
+            mx = P.max_abs_D_block(Dc, e.fl.prims, P.LVAL[cls[1]], P.LVAL[cls[3]])
+            want["K:" + cls] = want.get("K:" + cls, 0) + flops.count_k(e.fl, Dc, cls, mx)
+    for name, n in want.items():
+        assert got.get(name, (0, 0)) == (n, 0), (name, got.get(name), n)

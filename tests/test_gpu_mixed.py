@@ -43,3 +43,28 @@ def test_mixed_vs_oracle(dev, wl, policy):
     eng.precision = "fp64"
     J64, K64 = eng.build(D)
     assert np.abs(J64.cpu().numpy() - Jr).max() <= 1e-11 and np.abs(K64.cpu().numpy() - Kr).max() <= 1e-11
+
+
+def test_mixed_lattice64_energy_bar(dev):
+    """lattice64 (nbf 896) full J+K build: the engines' default mixed path
+    (FP32 below 1e-7, measured classes) against the plain FP64 path, which the
+    parity suite pins to the oracle at 1e-11.  |dE_J|, |dE_K| <= 1e-8 Eh, and
+    the quartet counters show that FP32 quartets were evaluated."""
+    from vlxmi.fock import FockEngine
+    print("case: lattice64 mixed (default threshold) vs FP64, energies", flush=True)
+    mol, D = configs.make_lattice64()
+    eng = FockEngine(mol, dev=dev)
+    Dd = dev.tensor(D)
+    J64, K64 = eng.build(Dd)
+    eng.precision = "mixed"
+    assert eng.fp32_threshold == 1e-7
+    J, K = eng.build(Dd)
+    n64, n32 = eng.quartet_counts()
+    EJ64, EK64 = eng.energies(Dd, J64, K64)
+    EJ, EK = eng.energies(Dd, J, K)
+    dJ, dK = float((J - J64).abs().max()), float((K - K64).abs().max())
+    print("  FP32 quartets %d of %d in mixed-kernel classes; dE_J %.2e dE_K %.2e max|dJ| %.2e max|dK| %.2e"
+          % (n32, n32 + n64, EJ - EJ64, EK - EK64, dJ, dK), flush=True)
+    assert n32 > 0 and n64 > 0
+    assert abs(EJ - EJ64) <= 1e-8 and abs(EK - EK64) <= 1e-8
+    assert dJ <= 1e-8 and dK <= 1e-8

@@ -2,8 +2,8 @@
 
 lattice4 (2x2x1 lattice, s3p2d1 basis, nbf 56): every J and K class against
 the full CPU oracle, 1e-11 max-abs (reference per-component screening).
-lattice64 (nbf 896): every J class on a sample of bra blocks, a set of K
-classes on a sample of (i,k) blocks, against the oracle on the same inputs; and
+lattice64 (nbf 896): every J class on a sample of bra blocks, every K class on
+a slice of (i,k) blocks, against the oracle on the same inputs; and
 size-independent properties of the full build (J, K symmetric, E_J, E_K > 0).
 omega != 0: range-separated K against the oracle.
 """
@@ -94,19 +94,30 @@ def test_lattice64_j_sampled(lat64):
         assert np.max(np.abs(g[idx] - r[idx])) <= TOL, (ab, cd)
 
 
-@pytest.mark.parametrize("cls", ["ssss", "spsp", "ppdd", "sdpd", "pddd", "dddd"])
-def test_lattice64_k_sampled(lat64, cls):
+@pytest.fixture(scope="module")
+def lat64_dens(lat64):
     mol, D, eng = lat64
-    _say("case: lattice64 K class %s (sampled (i,k) blocks)" % cls)
-    Dc_np = U.cart_density(mol, D)
-    g = eng.run_K(eng.cart_density(D), classes=[cls])[cls[0] + cls[2]].cpu().numpy()
-    ik = eng.fl.iklists[cls[0] + cls[2]]
-    rng = np.random.default_rng(6)
-    sel = np.unique(rng.choice(ik.nblk, size=min(ik.nblk, 6), replace=False)).astype(np.uint32)
-    r = U.oracle_matK(eng.fl, Dc_np, "component", ik_blocks=sel, classes=[cls])[cls[0] + cls[2]]
+    Dc = eng.cart_density(D)
+    return U.cart_density(mol, D), Dc, eng.max_D(Dc)
+
+
+@pytest.mark.parametrize("cls", P.k_classes())
+def test_lattice64_k_sampled(lat64, lat64_dens, cls):
+    """Every exchange class on a slice of (i,k) blocks (the reference's _local
+    share) against the full CSR lists, compared with the oracle."""
+    mol, D, eng = lat64
+    Dc_np, Dc, maxD = lat64_dens
+    ac = cls[0] + cls[2]
+    ik = eng.fl.iklists[ac]
+    rng = np.random.default_rng(600 + P.k_classes().index(cls))
+    sel = np.unique(np.concatenate([[0, ik.nblk - 1], rng.choice(ik.nblk, size=min(ik.nblk, 6), replace=False)]))
+    sel = sel.astype(np.uint32)
+    _say("case: lattice64 K class %s (%d sampled (i,k) blocks)" % (cls, sel.size))
+    g = eng.run_K(Dc, classes=[cls], maxD=maxD, ik_blocks={ac: sel})[ac].cpu().numpy()
+    r = U.oracle_matK(eng.fl, Dc_np, "component", ik_blocks=sel, classes=[cls])[ac]
     nac = P.NCART[P.LVAL[cls[0]]] * P.NCART[P.LVAL[cls[2]]]
-    idx = (sel[:, None] * nac + np.arange(nac)[None, :]).ravel()
-    assert np.max(np.abs(g[idx] - r[idx])) <= TOL
+    idx = (sel[:, None].astype(np.int64) * nac + np.arange(nac)[None, :]).ravel()
+    assert np.max(np.abs(g - r[idx])) <= TOL
 
 
 def test_lattice64_full_build_properties(lat64):
@@ -117,3 +128,14 @@ def test_lattice64_full_build_properties(lat64):
     assert np.max(np.abs(J - J.T)) <= 1e-10 and np.max(np.abs(K - K.T)) <= 1e-10
     EJ, EK = 0.5 * np.sum(D * J), 0.5 * np.sum(D * K)
     assert EJ > 0 and EK > 0 and EK < EJ
+
+
+def test_lattice64_fp64_bit_reproducible(lat64):
+    """Coulomb launches split their ket lists into chunks (grid.y) and combine
+    the chunks' partial sums in a fixed order, so two FP64 builds of the same
+    density give the same bits."""
+    mol, D, eng = lat64
+    _say("case: lattice64 FP64 build twice, bitwise")
+    J1, K1 = eng.build(D)
+    J2, K2 = eng.build(D)
+    assert eng.torch.equal(J1, J2) and eng.torch.equal(K1, K2)

@@ -5,6 +5,9 @@
 #include <cmath>
 #include <cstdint>
 #include <cstring>
+#include <map>
+#include <mutex>
+#include <utility>
 
 #include "vlxmi_device.hpp"
 
@@ -214,5 +217,115 @@ extern "C" int vlxmi_fill_zero(double* p, uint64_t n, void* stream)
     uint64_t blocks = (n + 255) / 256;
     if (blocks > 65536) blocks = 65536;
     hipLaunchKernelGGL(fill_zero_kernel, dim3((unsigned)blocks), dim3(256), 0, (hipStream_t)stream, p, n);
+    return (int)hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+// Ket-split partial sums of the Coulomb launchers.
+//
+// A Coulomb launch with ket_split = n > 1 runs n chunks of the ket list in
+// parallel (grid.y).  Chunk y writes its per-entry partial sums to
+// part[y * count + e]; ket_split_reduce_kernel then adds, for every entry e,
+// the n partials in chunk order with Neumaier's compensated summation, so the
+// result does not depend on the order the chunks ran in (no atomics: the build
+// is bit-reproducible) and carries about one rounding of the total instead of
+// one per chunk.
+// ---------------------------------------------------------------------------
+
+namespace {
+struct Workspace {
+    double* p = nullptr;
+    size_t n = 0;
+};
+std::mutex g_ws_mutex;
+std::map<std::pair<int, void*>, Workspace> g_ws;   // (device, stream) -> buffer
+}  // namespace
+
+// A device buffer of at least ndoubles doubles reserved for launches on
+// (current device, stream): launches on one stream run in order, so they can
+// share it; another stream gets its own.  Grown on demand (after synchronizing
+// the stream, whose earlier launches may still read the old buffer) and kept
+// until vlxmi_release_workspaces().
+extern "C" int vlxmi_ket_workspace(size_t ndoubles, void* stream, double** out)
+{
+    if (!out) return VLXMI_ERR_ARG;
+    int dev = 0;
+    hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess) return (int)e;
+    std::lock_guard<std::mutex> lock(g_ws_mutex);
+    Workspace& w = g_ws[std::make_pair(dev, stream)];
+    if (w.n < ndoubles) {
+        if (w.p) {
+            e = hipStreamSynchronize((hipStream_t)stream);
+            if (e != hipSuccess) return (int)e;
+            (void)hipFree(w.p);
+            w.p = nullptr;
+            w.n = 0;
+        }
+        const size_t n = ndoubles + ndoubles / 4;
+        e = hipMalloc((void**)&w.p, n * sizeof(double));
+        if (e != hipSuccess) {
+            w.p = nullptr;
+            return (int)e;
+        }
+        w.n = n;
+    }
+    *out = w.p;
+    return VLXMI_OK;
+}
+
+// Frees every ket-split workspace (after a device synchronization).
+extern "C" int vlxmi_release_workspaces(void)
+{
+    std::lock_guard<std::mutex> lock(g_ws_mutex);
+    if (!g_ws.empty()) {
+        hipError_t e = hipDeviceSynchronize();
+        if (e != hipSuccess) return (int)e;
+    }
+    for (auto& kv : g_ws)
+        if (kv.second.p) (void)hipFree(kv.second.p);
+    g_ws.clear();
+    return VLXMI_OK;
+}
+
+__device__ __forceinline__ void neumaier_add(double& s, double& c, double x)
+{
+    const double t = s + x;
+    c += (fabs(s) >= fabs(x)) ? ((s - t) + x) : ((x - t) + s);
+    s = t;
+}
+
+// 256 threads = 64 entries x 4 interleaved chunk subsets; the four partial
+// (sum, compensation) pairs of an entry are combined in a fixed order.
+__global__ void __launch_bounds__(256)
+ket_split_reduce_kernel(const double* __restrict__ part, uint32_t nsplit, uint32_t count,
+                        double* __restrict__ out)
+{
+    __shared__ double ls[4][64], lc[4][64];
+    const uint32_t el = threadIdx.x & 63, q = threadIdx.x >> 6;
+    const uint32_t e = blockIdx.x * 64 + el;
+    double s = 0.0, c = 0.0;
+    if (e < count)
+        for (uint32_t y = q; y < nsplit; y += 4) neumaier_add(s, c, part[(size_t)y * count + e]);
+    ls[q][el] = s;
+    lc[q][el] = c;
+    __syncthreads();
+    if (q == 0 && e < count) {
+        double S = ls[0][el], Cc = lc[0][el];
+        for (int k = 1; k < 4; ++k) {
+            neumaier_add(S, Cc, ls[k][el]);
+            Cc += lc[k][el];
+        }
+        out[e] += S + Cc;
+    }
+}
+
+// out[e] += sum_{y < nsplit} part[y * count + e], e < count (see above).
+extern "C" int vlxmi_ket_split_reduce(const double* part, uint32_t nsplit, uint32_t count, double* out, void* stream)
+{
+    if (count == 0 || nsplit == 0) return VLXMI_OK;
+    if (!part || !out) return VLXMI_ERR_ARG;
+    hipLaunchKernelGGL(ket_split_reduce_kernel, dim3((count + 63) / 64), dim3(256), 0, (hipStream_t)stream, part,
+                       nsplit, count, out);
     return (int)hipGetLastError();
 }

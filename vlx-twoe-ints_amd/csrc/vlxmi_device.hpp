@@ -40,6 +40,11 @@
 
 #include "vlxmi_status.h"
 
+// Ket-split partial sums of the Coulomb launchers (vlxmi_capi.cpp).
+extern "C" int vlxmi_ket_workspace(size_t ndoubles, void* stream, double** out);
+extern "C" int vlxmi_ket_split_reduce(const double* part, uint32_t nsplit, uint32_t count, double* out,
+                                      void* stream);
+
 #define VLX_BOYS_NCOL 16
 #define VLX_BOYS_NROW 2001
 #define VLX_BOYS_DT 0.02

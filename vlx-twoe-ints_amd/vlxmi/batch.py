@@ -295,6 +295,10 @@ class Pack:
                               pair_data=t.zeros_like(j["pair_data"]), mat_Q=t.zeros_like(j["mat_Q"]),
                               mat_D=t.zeros_like(j["pair_data"]), count=j["count"]) for pc, j in self.jl.items()}
         self.maxD = t.zeros((9, max(nm, 1)), dtype=t.float64, device=dv)
+        # per class launch (J classes in j_classes() order, then K classes): primitive
+        # quartets evaluated in FP64 / FP32 by the last build, all molecules together
+        self.launch_names = (["J:" + ab + cd for ab, cd in P.j_classes()] + ["K:" + c for c in P.k_classes()])
+        self.qcount = t.zeros((len(self.launch_names), 2), dtype=t.int64, device=dv)
 
 
 def _pack_assembler(pack, dev, class_off, lists, plan_attr):
@@ -333,7 +337,7 @@ def _pack_assembler(pack, dev, class_off, lists, plan_attr):
 
 
 class BatchEngine:
-    def __init__(self, dev: Device = None, thr=1e-12, precision="fp64", fp32_threshold=1e-6, omega=0.0,
+    def __init__(self, dev: Device = None, thr=1e-12, precision="fp64", fp32_threshold=1e-7, omega=0.0,
                  mixed_policy="measured"):
         self.dev = dev or Device()
         self.torch = self.dev.torch
@@ -348,14 +352,19 @@ class BatchEngine:
     # ------------------------------------------------------------------ helpers
     def density_flat(self, pack: Pack, densities):
         t = self.torch
-        parts = []
+        parts, stats = [], []
         for e, D in zip(pack.entries, densities):
-            Dd = D if t.is_tensor(D) else self.dev.tensor(np.asarray(D, dtype=np.float64))
-            if not e.identity_T:
-                Dd = e.T.t() @ Dd @ e.T
+            Ds = D if t.is_tensor(D) else self.dev.tensor(np.asarray(D, dtype=np.float64))
+            Dd = Ds if e.identity_T else e.T.t() @ Ds @ e.T
+            stats.append(t.stack([Dd.abs().amax(), (Ds - Ds.t()).abs().amax(), Ds.abs().amax()]))
             # exactly symmetric: some exchange classes read D_jl as D[l][j]
             Dd = 0.5 * (Dd + Dd.t())
             parts.append(Dd.reshape(-1))
+        # one host read for the whole batch: every density must fit the bound its
+        # molecule's lists were screened for and be symmetric to rounding
+        if stats:
+            for e, st in zip(pack.entries, t.stack(stats).cpu().tolist()):
+                P.check_density(e.fl, *st)
         return t.cat(parts).contiguous()
 
     def _maxD(self, pack, Dflat):
@@ -409,7 +418,7 @@ class BatchEngine:
         kets = pack.kets
         matJ, matK = pack.matJ, pack.matK
         maxD = pack.maxD
-        for buf in (pack.jbuf, pack.kbuf, pack.Jflat, pack.Kflat):
+        for buf in (pack.jbuf, pack.kbuf, pack.Jflat, pack.Kflat, pack.qcount):
             zero_(self.dev, buf)
         st = self.dev.stream_ptr()
         for (ab, cd), (items, cost) in pack.jwork.items():
@@ -428,8 +437,8 @@ class BatchEngine:
             work = tun.get("work", pack.jwork_dev[(ab, cd)])
             args += [work.data_ptr(), len(items)]
             mixed = use_mixed("J", cls, self.precision, self.mixed_policy)
-            if mixed:
-                args.append(self.fp32_threshold)
+            qc = pack.qcount[pack.launch_names.index("J:" + cls)].data_ptr()
+            args += [self.fp32_threshold, qc] if mixed else [qc]
             args += [tun.get("wg", 0), st]
             name = "computeCoulombFock" + cls.upper() + "_batched" + ("_mixed" if mixed else "")
             self._ev(name, getattr(L, "vlxmi_launch_" + name), *args)
@@ -457,13 +466,19 @@ class BatchEngine:
             args += [blk_list.data_ptr(), ik["nblk"], ik["blk_mol"].data_ptr(), maxD[bdi].data_ptr(),
                      pack.mol_D_off.data_ptr(), pack.mol_naos.data_ptr(), pack.mol_ao_off.data_ptr()]
             mixed = use_mixed("K", cls, self.precision, self.mixed_policy)
-            if mixed:
-                args.append(self.fp32_threshold)
+            qc = pack.qcount[pack.launch_names.index("K:" + cls)].data_ptr()
+            args += [self.fp32_threshold, qc] if mixed else [qc]
             args += [tun.get("wg", 0), st]
             name = "computeExchangeFock" + cls.upper() + "_batched" + ("_mixed" if mixed else "")
             self._ev(name, getattr(L, "vlxmi_launch_" + name), *args)
         pack.jasm.run(pack.jbuf, pack.Jflat)
         pack.kasm.run(pack.kbuf, pack.Kflat)
+
+    def quartet_counts(self, pack: Pack):
+        """{launch name ("J:abcd" / "K:abcd"): (FP64, FP32) primitive quartets}
+        of the last build of ``pack`` (reads the device)."""
+        c = pack.qcount.cpu().tolist()
+        return {n: (int(a), int(b)) for n, (a, b) in zip(pack.launch_names, c) if a or b}
 
     def unpack(self, pack: Pack, flat):
         out = []

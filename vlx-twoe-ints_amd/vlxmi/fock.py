@@ -34,7 +34,10 @@ from .runtime import Device, check, lib
 
 class FockEngine:
     def __init__(self, mol, thr=1e-12, dev=None, d_bound=100.0, omega=0.0, target_waves=32768,
-                 precision="fp64", fp32_threshold=1e-6, mixed_policy="measured"):
+                 precision="fp64", fp32_threshold=1e-7, mixed_policy="measured", lists=None):
+        """``lists``: the molecule's FockLists if already built (e.g. by a
+        batched-engine entry of the same molecule); otherwise they are built
+        here with the GPU Schwarz kernels."""
         self.dev = dev or Device()
         t = self.dev.torch
         self.torch = t
@@ -51,8 +54,15 @@ class FockEngine:
         self.fp32_threshold = float(fp32_threshold)
         # which classes launch their _mixed kernel in mixed precision (vlxmi.policy)
         self.mixed_policy = mixed_policy
+        # device int64[2]: primitive quartets the _mixed kernels evaluated in FP64 / FP32
+        # since the last reset_counts() (every mixed build() resets it first)
+        self.qcount = None
         t0 = time.time()
-        self.fl = P.build_lists(mol, self.dev.schwarz, thr=thr, d_bound=d_bound)
+        if lists is not None:
+            assert lists.mol is mol and lists.thr == self.thr
+            self.fl = lists
+        else:
+            self.fl = P.build_lists(mol, self.dev.schwarz, thr=thr, d_bound=d_bound)
         self.setup_lists_s = time.time() - t0
         fl = self.fl
         d = self.dev
@@ -149,6 +159,10 @@ class FockEngine:
         if not t.is_tensor(D_sph):
             D_sph = self.dev.tensor(np.asarray(D_sph, dtype=np.float64))
         Dc = self.T.t() @ D_sph @ self.T
+        # one device reduction, one host read: the density must fit the bound
+        # the lists were screened for and be symmetric to rounding
+        st = t.stack([Dc.abs().amax(), (D_sph - D_sph.t()).abs().amax(), D_sph.abs().amax()]).cpu().tolist()
+        P.check_density(self.fl, *st)
         # exactly symmetric: some exchange classes read D_jl as D[l][j]
         return (0.5 * (Dc + Dc.t())).contiguous()
 
@@ -194,16 +208,46 @@ class FockEngine:
         s = max(1, -(-self.target_waves // tiles))
         return int(min(s, max(1, nket_blocks // 64)))
 
-    def run_J(self, Dc, kets=None, classes=None):
-        """Per bra class mat_J (bra list order), summed over ket classes."""
+    def bra_slice(self, ab, blocks):
+        """The bra list of pair class ``ab`` restricted to the primitive-pair
+        blocks ``blocks`` (indices into the resident list): the reference's
+        ``_local`` arguments (gen_coulomb.py:81-86), the bra share one GPU of
+        several would take.  The ket list stays whole."""
+        t = self.torch
+        j = self.jl[ab]
+        nab = j["nab"]
+        b = t.as_tensor(np.asarray(blocks, dtype=np.int64), device=self.dev.device)
+        ent = (b[:, None] * nab + t.arange(nab, device=self.dev.device)[None, :]).reshape(-1)
+        return dict(first=j["first"][ent].contiguous(), second=j["second"][ent].contiguous(),
+                    pair_data=j["pair_data"][ent].contiguous(), mat_Q=j["mat_Q"][ent].contiguous(),
+                    count=int(ent.numel()), nblk=int(b.numel()), nab=nab)
+
+    def ik_slice(self, ac, blocks):
+        """The (i,k) list of class ``ac`` restricted to the (I,K) blocks
+        ``blocks`` (the exchange share of one GPU of several)."""
+        t = self.torch
+        ik = self.ik[ac]
+        nac = ik["count"] // max(ik["nblk"], 1)
+        b = t.as_tensor(np.asarray(blocks, dtype=np.int64), device=self.dev.device)
+        ent = (b[:, None] * nac + t.arange(nac, device=self.dev.device)[None, :]).reshape(-1)
+        return dict(i=ik["i"][ent].contiguous(), k=ik["k"][ent].contiguous(), count=int(ent.numel()),
+                    nblk=int(b.numel()))
+
+    def run_J(self, Dc, kets=None, classes=None, bra_blocks=None):
+        """Per bra class mat_J (bra list order), summed over ket classes.
+        ``bra_blocks`` ({ab: block indices}) runs each bra class on that slice of
+        its list only; mat_J then follows the slice's order."""
         t = self.torch
         L = lib()
         kets = kets or self.prepare_kets(Dc)
         out = {}
+        slices = {}
         for ab, cd in (classes or P.j_classes()):
             if ab not in self.jl or cd not in self.jl:
                 continue
-            bra, ket = self.jl[ab], kets[cd]
+            if bra_blocks is not None and ab not in slices:
+                slices[ab] = self.bra_slice(ab, bra_blocks[ab])
+            bra, ket = slices.get(ab, self.jl[ab]), kets[cd]
             if ab not in out:
                 out[ab] = t.zeros(bra["count"], dtype=t.float64, device=Dc.device)
             cls = ab + cd
@@ -220,7 +264,7 @@ class FockEngine:
             mixed = use_mixed("J", cls, self.precision, self.mixed_policy)
             name = "computeCoulombFock" + cls.upper() + ("_mixed" if mixed else "")
             if mixed:
-                args.append(self.fp32_threshold)
+                args += [self.fp32_threshold, self._qcount_ptr()]
             fn = getattr(L, "vlxmi_launch_" + name)
             ev = self._ev_start()
             check(fn(*args, split, self.dev.stream_ptr()), name)
@@ -237,17 +281,23 @@ class FockEngine:
                 out["spd"[lb] + "spd"[ld]] = float(Dc[ab][:, ad].abs().max().item())
         return out
 
-    def run_K(self, Dc, classes=None, maxD=None):
+    def run_K(self, Dc, classes=None, maxD=None, ik_blocks=None):
+        """Per (a,c) class mat_K ((i,k) list order), summed over (b,d).
+        ``ik_blocks`` ({ac: block indices}) runs each class on that slice of its
+        (i,k) list only; the CSR lists stay whole."""
         t = self.torch
         L = lib()
         maxD = maxD or self.max_D(Dc)
         out = {}
+        slices = {}
         for cls in (classes or P.k_classes()):
             a, b, c, d = cls
             ac, ab, cd, bd = a + c, a + b, c + d, b + d
             if ac not in self.ik or ab not in self.kc or cd not in self.kc:
                 continue
-            ik = self.ik[ac]
+            if ik_blocks is not None and ac not in slices:
+                slices[ac] = self.ik_slice(ac, ik_blocks[ac])
+            ik = slices.get(ac, self.ik[ac])
             if ac not in out:
                 out[ac] = t.zeros(ik["count"], dtype=t.float64, device=Dc.device)
             args = [out[ac].data_ptr(), ik["i"].data_ptr(), ik["k"].data_ptr(), ik["count"]]
@@ -265,7 +315,7 @@ class FockEngine:
             mixed = use_mixed("K", cls, self.precision, self.mixed_policy)
             name = "computeExchangeFock" + cls.upper() + ("_mixed" if mixed else "")
             if mixed:
-                args.append(self.fp32_threshold)
+                args += [self.fp32_threshold, self._qcount_ptr()]
             fn = getattr(L, "vlxmi_launch_" + name)
             ev = self._ev_start()
             check(fn(*args, self.dev.stream_ptr()), name)
@@ -286,8 +336,32 @@ class FockEngine:
                                            self.thr / maxD, out.data_ptr(), self.dev.stream_ptr()), "count_k_quartets")
         return int(out.item())
 
+    def reset_counts(self):
+        """Zero the FP64/FP32 quartet counters of the _mixed kernels."""
+        t = self.torch
+        if self.qcount is None:
+            self.qcount = t.zeros(2, dtype=t.int64, device=self.dev.device)
+        else:
+            self.qcount.zero_()
+
+    def _qcount_ptr(self):
+        if self.qcount is None:
+            self.reset_counts()
+        return self.qcount.data_ptr()
+
+    def quartet_counts(self):
+        """(FP64, FP32) primitive quartets evaluated by the _mixed kernels since
+        the last reset (a mixed build() resets first); classes that ran their
+        plain FP64 kernel are not counted.  Reads the device (synchronizes)."""
+        if self.qcount is None:
+            return 0, 0
+        n64, n32 = self.qcount.cpu().tolist()
+        return int(n64), int(n32)
+
     def build(self, D_sph, want_k=True):
         """J and K in the output basis for density D (device tensors)."""
+        if self.precision == "mixed":
+            self.reset_counts()
         Dc = self.cart_density(D_sph)
         mJ = self.run_J(Dc)
         Jc = self.assemble_J(mJ)

@@ -130,8 +130,10 @@ def header_text():
          " * computeExchangeFock{A}{B}{C}{D} (gen_exchange.py:75-115,",
          " * new_exchange_code/exchangefockwriter.py:55-104), all pointers device",
          " * pointers owned by the caller, outputs accumulated (+=), followed by launch",
-         " * controls (J: ket_split = number of ket-list chunks run in parallel; >1 makes",
-         " * the accumulation atomic) and the hipStream_t as void*.  They replace the",
+         " * controls (J: ket_split = number of ket-list chunks run in parallel; with >1",
+         " * each chunk's partial sums go to a workspace the library keeps per (device,",
+         " * stream) and are added to mat_J in chunk order by a compensated reduction,",
+         " * so results are bit-reproducible) and the hipStream_t as void*.  They replace the",
          " * kernel launches of VeloxChem's GPU Fock driver (src/gpu/EriCoulomb.cu /",
          " * EriExchange.cu, named at new_exchange_code/erisplitcounter.py:44).",
          " * Return: 0 ok, <0 argument/layout error (vlxmi_status.h), >0 hipError_t.",
@@ -160,27 +162,30 @@ def header_text():
         o.append("")
     for ab, cd in j_classes():
         cls = j_name(ab, cd)
-        sig = emit._strip_restrict(emit.j_signature(cls)) + ["const double fp32_threshold"]
+        sig = emit._strip_restrict(emit.j_signature(cls) + emit.MIXED_ARGS)
         o.append("/* mixed-precision variant of computeCoulombFock%s (no counterpart in the reference): quartets" % cls.upper())
         o.append(" * whose bound Qmax_IJ*max|Q_kl D_kl| is below fp32_threshold for every lane of a wavefront run the")
-        o.append(" * VRR/HRR in FP32; Boys, prefactors and accumulation stay FP64 */")
+        o.append(" * VRR/HRR in FP32; Boys, prefactors and accumulation stay FP64.  quartet_counts (may be NULL):")
+        o.append(" * [0] += primitive quartets evaluated in FP64, [1] += in FP32 (device, caller-zeroed) */")
         o.append("int vlxmi_launch_computeCoulombFock%s_mixed(%s,\n    uint32_t ket_split, void* stream);" % (cls.upper(), ",\n    ".join(sig)))
         o.append("")
     for cls in k_classes():
-        sig = emit._strip_restrict(emit.k_signature(cls)) + ["const double fp32_threshold"]
-        o.append("/* mixed-precision variant of computeExchangeFock%s (no counterpart in the reference) */" % cls.upper())
+        sig = emit._strip_restrict(emit.k_signature(cls) + emit.MIXED_ARGS)
+        o.append("/* mixed-precision variant of computeExchangeFock%s (no counterpart in the reference); bound" % cls.upper())
+        o.append(" * QKmax_IJ*QKmax_KL*maxD_bd, quartet_counts as for the Coulomb variants */")
         o.append("int vlxmi_launch_computeExchangeFock%s_mixed(%s,\n    void* stream);" % (cls.upper(), ",\n    ".join(sig)))
         o.append("")
     o.append("/* Batched (persistent) variants: one launch covers many molecules whose lists are")
     o.append(" * concatenated.  J: work = n items of 4 uint32 (bra block begin, end <= begin+64, ket block")
     o.append(" * begin, end); each bra tile in at most one item.  K: blk_list = (i,k) blocks to process,")
     o.append(" * blk_mol = molecule of every block, mol_maxD/mol_D_off/mol_naos/mol_ao_off per molecule.")
+    o.append(" * quartet_counts (may be NULL): [0] += primitive quartets evaluated in FP64, [1] += in FP32.")
     o.append(" * n_workgroups = persistent grid size (0: one per item).  No counterpart in the reference. */")
     for mixed in (False, True):
         for ab, cd in j_classes():
             cls = j_name(ab, cd)
-            sig = emit._strip_restrict(emit.j_signature(cls)) + ["const uint32_t* work", "const uint32_t nwork"] \
-                + (["const double fp32_threshold"] if mixed else [])
+            sig = emit._strip_restrict(emit.j_signature(cls) + ["const uint32_t* work", "const uint32_t nwork"]
+                                       + emit.tail_args(mixed, True))
             o.append("int vlxmi_launch_computeCoulombFock%s_batched%s(%s,\n    uint32_t n_workgroups, void* stream);"
                      % (cls.upper(), "_mixed" if mixed else "", ",\n    ".join(sig)))
             o.append("")
@@ -188,7 +193,7 @@ def header_text():
             sig = emit._strip_restrict(emit.k_signature(cls)) + [
                 "const uint32_t* blk_list", "const uint32_t nlist", "const uint32_t* blk_mol",
                 "const double* mol_maxD", "const unsigned long long* mol_D_off", "const uint32_t* mol_naos",
-                "const uint32_t* mol_ao_off"] + (["const double fp32_threshold"] if mixed else [])
+                "const uint32_t* mol_ao_off"] + emit._strip_restrict(emit.tail_args(mixed, True))
             o.append("int vlxmi_launch_computeExchangeFock%s_batched%s(%s,\n    uint32_t n_workgroups, void* stream);"
                      % (cls.upper(), "_mixed" if mixed else "", ",\n    ".join(sig)))
             o.append("")

@@ -227,10 +227,12 @@ def _dag_lines(dag):
     return out
 
 
-def ket_loop(ind, act, screen, bound, header, body, ty, check_lo, lo):
+def ket_loop(ind, act, screen, bound, header, body, ty, check_lo, lo, count=False):
     """One loop over the staged records from the current ``s`` to nt-1, the
     recursion in type ``ty``; with ``check_lo`` it stops (setting ``lo``) at
-    the first record where every live lane's bound is below fp32_threshold."""
+    the first record where every live lane's bound is below fp32_threshold.
+    With ``count`` every evaluated quartet adds one to the lane's counter of
+    its precision (nq64 / nq32)."""
     q = [ind + "for (; s < nt; ++s) {",
          ind + "    const double* R = ket + s * REC;",
          ind + "    if (%s && %s) %s = false;" % (act, screen, act),
@@ -238,13 +240,15 @@ def ket_loop(ind, act, screen, bound, header, body, ty, check_lo, lo):
     if check_lo:
         q.append(ind + "    if (__all(!%s || %s < fp32_threshold)) { %s = true; break; }" % (act, bound, lo))
     q.append(ind + "    if (%s) {" % act)
+    if count:
+        q.append(ind + "        %s += 1u;" % ("nq32" if ty == "float" else "nq64"))
     q += [ind + "        " + x for x in header]
     q += body(ty, ind + "        ")
     q += [ind + "    }", ind + "}"]
     return q
 
 
-def ket_loops(ind, act, screen, bound, header, body, mixed, lo):
+def ket_loops(ind, act, screen, bound, header, body, mixed, lo, count=None):
     """Loop(s) over the staged ket records s = 0..nt-1 of one tile.
 
     ``act`` is the lane's live flag, cleared by the screening test ``screen``
@@ -256,8 +260,10 @@ def ket_loops(ind, act, screen, bound, header, body, mixed, lo):
     first record where that holds and one FP32 loop after it; the wave-uniform
     flag ``lo`` carries the switch to later tiles.  Each loop then holds one
     recursion body only."""
+    count = mixed if count is None else count
+
     def loop(ty, check_lo):
-        return ket_loop(ind, act, screen, bound, header, body, ty, check_lo, lo)
+        return ket_loop(ind, act, screen, bound, header, body, ty, check_lo, lo, count=count)
 
     o = [ind + "{", ind + "uint32_t s = 0;"]
     if mixed:
@@ -271,6 +277,36 @@ def ket_loops(ind, act, screen, bound, header, body, mixed, lo):
         o += loop("double", False)
     o.append(ind + "}")
     return o
+
+
+def count_decl():
+    """Per-lane quartet counters of a mixed kernel (FP64 and FP32 evaluations)."""
+    return ["    uint32_t nq64 = 0, nq32 = 0;"]
+
+
+def count_flush(ind="    "):
+    """Wavefront sums of the counters, added by lane 0 to quartet_counts[0..1]
+    (FP64, FP32 primitive quartets) when the caller passed a counter array."""
+    return [ind + "if (quartet_counts != nullptr) {",
+            ind + "    #pragma unroll",
+            ind + "    for (int off = 32; off > 0; off >>= 1) { nq64 += __shfl_xor(nq64, off, 64); nq32 += __shfl_xor(nq32, off, 64); }",
+            ind + "    if (threadIdx.x == 0) {",
+            ind + "        atomicAdd(&quartet_counts[0], (unsigned long long)nq64);",
+            ind + "        atomicAdd(&quartet_counts[1], (unsigned long long)nq32);",
+            ind + "    }",
+            ind + "}"]
+
+
+MIXED_ARGS = ["const double fp32_threshold", "unsigned long long* __restrict__ quartet_counts"]
+
+
+def tail_args(mixed, batched):
+    """Parameters after the reference list (and the batched work-list ones):
+    mixed kernels take fp32_threshold and quartet_counts, plain batched kernels
+    quartet_counts (their FP64 quartets are counted too)."""
+    if mixed:
+        return list(MIXED_ARGS)
+    return [MIXED_ARGS[1]] if batched else []
 
 
 KET_HEADER = ["const double eta = R[1], inv_eta = R[2], sq_eta = R[12];",
@@ -369,11 +405,21 @@ def emit_j(cls, mixed=False, batched=False, kt=None, layout="reg", lds_boys=Fals
     rec += rec % 2
     name = "computeCoulombFock" + cls.upper() + ("_batched" if batched else "") + ("_mixed" if mixed else "")
     sig = j_signature(cls) + (["const uint32_t* __restrict__ work", "const uint32_t nwork"] if batched else []) \
-        + (["const double fp32_threshold"] if mixed else [])
+        + tail_args(mixed, batched)
     ty = "double"
     o = []
-    o.append("__global__ void __launch_bounds__(TILE_SIZE_J)")
-    o.append("%s(%s)" % (name, ",\n    ".join(sig)))
+    # Non-batched kernels share one body: with ket_partial == nullptr the
+    # chunks of a ket split (grid.y) add their sums to mat_J atomically (the
+    # kernel with the reference's parameter list); otherwise chunk y stores its
+    # partial sums at ket_partial[y * count_local + e] for the launcher's
+    # deterministic reduction.
+    KP = ["double* __restrict__ ket_partial"]
+    if batched:
+        o.append("__global__ void __launch_bounds__(TILE_SIZE_J)")
+        o.append("%s(%s)" % (name, ",\n    ".join(sig)))
+    else:
+        o.append("static __device__ __forceinline__ void")
+        o.append("%s_body(%s)" % (name, ",\n    ".join(sig + KP)))
     o.append("{")
     o.append("    constexpr int NA = %d, NB = %d, NC = %d, ND = %d, NAB = %d, NCD = %d;" % (na, nb, nc, nd, nab, ncd))
     o.append("    constexpr int KT = %d, REC = %d;" % (LDS_BOYS_KT if lds_boys else (kt or KET_TILE), rec))
@@ -384,6 +430,7 @@ def emit_j(cls, mixed=False, batched=False, kt=None, layout="reg", lds_boys=Fals
     o.append("    const uint32_t nbra = %s_prim_pair_count_local / NAB;" % ab)
     o.append("    const uint32_t nket = %s_prim_pair_count / NCD;" % cd)
     if batched:
+        o += count_decl()
         o.append("    for (uint32_t w = blockIdx.x; w < nwork; w += gridDim.x) {")
         o.append("    const uint32_t bra_end = work[4 * w + 1] < nbra ? work[4 * w + 1] : nbra;")
         o.append("    const uint32_t bb = work[4 * w] + lane;")
@@ -439,6 +486,8 @@ def emit_j(cls, mixed=False, batched=False, kt=None, layout="reg", lds_boys=Fals
     o.append("    }")
     if mixed:
         o.append("    bool lo_mode = false;")
+        if not batched:
+            o += count_decl()
     o.append("    for (uint32_t t0 = k_begin; t0 < k_end; t0 += KT) {")
     o.append("        if (!__any(active)) break;")
     o.append("        __syncthreads();")
@@ -510,7 +559,8 @@ def emit_j(cls, mixed=False, batched=False, kt=None, layout="reg", lds_boys=Fals
     # the FP32 loop of a mixed kernel may run fewer passes (half-size values)
     jsl32 = plan_slices_f32(la, lb, lc, ld, "J") if mixed else jsl
     o += ket_loops("        ", "active", "qmax * R[0] * (1.0 + 1e-13) <= thr", "qmax * R[0]", KET_HEADER,
-                   lambda ty, ind: jbody(ty, ind, jsl if ty == "double" else jsl32), mixed, "lo_mode")
+                   lambda ty, ind: jbody(ty, ind, jsl if ty == "double" else jsl32), mixed, "lo_mode",
+                   count=mixed or batched)
     o.append("    }")
     o.append("    if (own) {")
     o.append("        const uint32_t e0 = bb * NAB;")
@@ -520,16 +570,33 @@ def emit_j(cls, mixed=False, batched=False, kt=None, layout="reg", lds_boys=Fals
     if batched:
         o.append("            mat_J[e0 + t] += v;")
     else:
-        o.append("            if (gridDim.y > 1) atomicAdd(&mat_J[e0 + t], v); else mat_J[e0 + t] += v;")
+        o.append("            if (ket_partial != nullptr) ket_partial[(size_t)blockIdx.y * %s_prim_pair_count_local + e0 + t] = v;" % ab)
+        o.append("            else if (gridDim.y > 1) atomicAdd(&mat_J[e0 + t], v); else mat_J[e0 + t] += v;")
     o.append("        }")
     o.append("    }")
     if batched:
         o.append("    }")
+    if mixed or batched:
+        o += count_flush()
     o.append("}")
     o.append("")
-    # launcher
     lsig = _strip_restrict(sig)
     args = _argnames(sig)
+    if not batched:
+        if not mixed:
+            o.append("__global__ void __launch_bounds__(TILE_SIZE_J)")
+            o.append("%s(%s)" % (name, ",\n    ".join(sig)))
+            o.append("{")
+            o.append("    %s_body(%s, nullptr);" % (name, ", ".join(args)))
+            o.append("}")
+            o.append("")
+        o.append("__global__ void __launch_bounds__(TILE_SIZE_J)")
+        o.append("%s_ks(%s)" % (name, ",\n    ".join(sig + KP)))
+        o.append("{")
+        o.append("    %s_body(%s, ket_partial);" % (name, ", ".join(args)))
+        o.append("}")
+        o.append("")
+    # launcher
     if batched:
         o.append('extern "C" int vlxmi_launch_%s(%s,\n    uint32_t n_workgroups, void* stream)' % (name, ",\n    ".join(lsig)))
     else:
@@ -548,8 +615,26 @@ def emit_j(cls, mixed=False, batched=False, kt=None, layout="reg", lds_boys=Fals
         o.append("    if (ket_split == 0) ket_split = 1;")
         o.append("    if (ket_split > nket) ket_split = nket;")
         o.append("    dim3 grid((nbra + TILE_SIZE_J - 1) / TILE_SIZE_J, ket_split);")
-    o.append("    hipLaunchKernelGGL(%s, grid, dim3(TILE_SIZE_J), 0, (hipStream_t)stream, %s);" % (name, ", ".join(args)))
-    o.append("    return (int)hipGetLastError();")
+    if batched:
+        o.append("    hipLaunchKernelGGL(%s, grid, dim3(TILE_SIZE_J), 0, (hipStream_t)stream, %s);" % (name, ", ".join(args)))
+        o.append("    return (int)hipGetLastError();")
+    else:
+        # ket split: per-chunk partial sums in the stream's workspace, then a
+        # compensated reduction in chunk order (deterministic, no atomics)
+        o.append("    if (ket_split > 1) {")
+        o.append("        double* part = nullptr;")
+        o.append("        int rc = vlxmi_ket_workspace((size_t)ket_split * %s_prim_pair_count_local, stream, &part);" % ab)
+        o.append("        if (rc != 0) return rc;")
+        o.append("        hipLaunchKernelGGL(%s_ks, grid, dim3(TILE_SIZE_J), 0, (hipStream_t)stream, %s, part);" % (name, ", ".join(args)))
+        o.append("        rc = (int)hipGetLastError();")
+        o.append("        if (rc != 0) return rc;")
+        o.append("        return vlxmi_ket_split_reduce(part, ket_split, %s_prim_pair_count_local, mat_J, stream);" % ab)
+        o.append("    }")
+        if mixed:
+            o.append("    hipLaunchKernelGGL(%s_ks, grid, dim3(TILE_SIZE_J), 0, (hipStream_t)stream, %s, nullptr);" % (name, ", ".join(args)))
+        else:
+            o.append("    hipLaunchKernelGGL(%s, grid, dim3(TILE_SIZE_J), 0, (hipStream_t)stream, %s);" % (name, ", ".join(args)))
+        o.append("    return (int)hipGetLastError();")
     o.append("}")
     return "\n".join(o) + "\n"
 
@@ -586,7 +671,7 @@ def emit_k(cls, mixed=False, batched=False, kt=None, layout="reg", dorder="jl", 
     bsig = ["const uint32_t* __restrict__ blk_list", "const uint32_t nlist", "const uint32_t* __restrict__ blk_mol",
             "const double* __restrict__ mol_maxD", "const unsigned long long* __restrict__ mol_D_off",
             "const uint32_t* __restrict__ mol_naos", "const uint32_t* __restrict__ mol_ao_off"]
-    sig = k_signature(cls) + (bsig if batched else []) + (["const double fp32_threshold"] if mixed else [])
+    sig = k_signature(cls) + (bsig if batched else []) + tail_args(mixed, batched)
     ty = "double"
     o = []
     o.append("__global__ void __launch_bounds__(TILE_SIZE_K)")
@@ -617,6 +702,8 @@ def emit_k(cls, mixed=False, batched=False, kt=None, layout="reg", dorder="jl", 
     o.append("    const uint32_t lane = threadIdx.x;")
     if lds_boys:
         o += lds_boys_setup(L)
+    if mixed or batched:
+        o += count_decl()
     if batched:
         o.append("    (void)%s_max_D;" % bd)
         o.append("    for (uint32_t w = blockIdx.x; w < nlist; w += gridDim.x) {")
@@ -847,7 +934,8 @@ def emit_k(cls, mixed=False, batched=False, kt=None, layout="reg", dorder="jl", 
                 o.append("            bool lo%d = lo_mode;" % isl)
             o += pacc_io(group, False)
             o += ket_loops("            ", "act%d" % isl, kscreen, kbound, KET_HEADER,
-                           lambda ty, ind, g=group: kbody(ty, ind, [g]), mixed, "lo%d" % isl)
+                           lambda ty, ind, g=group: kbody(ty, ind, [g]), mixed, "lo%d" % isl,
+                           count=(mixed or batched) and isl == 0)
             o += pacc_io(group, True)
             o.append("            act_end = act%d;" % isl)
             if mixed:
@@ -871,7 +959,7 @@ def emit_k(cls, mixed=False, batched=False, kt=None, layout="reg", dorder="jl", 
             o.append("            if (!lo%d) {" % isl)
             o += ["    " + x for x in ket_loop("            ", "act%d" % isl, kscreen, kbound, KET_HEADER,
                                              lambda ty, ind, g=group: kbody(ty, ind, [g]), "double", True,
-                                             "lo%d" % isl)]
+                                             "lo%d" % isl, count=isl == 0)]
             o.append("            }")
             o += pacc_io(group, True)
             o.append("            s_sw = s; act_sw = act%d; lo_sw = lo%d;" % (isl, isl))
@@ -884,7 +972,7 @@ def emit_k(cls, mixed=False, batched=False, kt=None, layout="reg", dorder="jl", 
             o += pacc_io(group, False)
             o.append("            uint32_t s = s_sw;")
             o += ket_loop("            ", "actf%d" % isl, kscreen, kbound, KET_HEADER,
-                          lambda ty, ind, g=group: kbody(ty, ind, [g]), "float", False, "lo_sw")
+                          lambda ty, ind, g=group: kbody(ty, ind, [g]), "float", False, "lo_sw", count=isl == 0)
             o += pacc_io(group, True)
             o.append("            act_end = actf%d;" % isl)
             o.append("            }")
@@ -893,7 +981,8 @@ def emit_k(cls, mixed=False, batched=False, kt=None, layout="reg", dorder="jl", 
         o.append("            lo_mode = lo_sw;")
     else:
         o += ket_loops("            ", "active", kscreen, kbound, KET_HEADER,
-                       lambda ty, ind: kbody(ty, ind, [list(range(nc))]), mixed, "lo_mode")
+                       lambda ty, ind: kbody(ty, ind, [list(range(nc))]), mixed, "lo_mode",
+                       count=mixed or batched)
     o.append("        }")
     o.append("    }")
     o.append("    #pragma unroll")
@@ -906,6 +995,8 @@ def emit_k(cls, mixed=False, batched=False, kt=None, layout="reg", dorder="jl", 
     if batched:
         o.append("    __syncthreads();")
         o.append("    }")
+    if mixed or batched:
+        o += count_flush()
     o.append("}")
     o.append("")
     lsig = _strip_restrict(sig)

@@ -233,6 +233,35 @@ class FockLists:
         return self.mol.naos_cart
 
 
+class DensityError(ValueError):
+    """A density the resident lists cannot serve exactly (see check_density)."""
+
+
+# relative asymmetry |D - D^T|_max / max|D| accepted as rounding of a symmetric density
+SYM_RTOL = 1e-12
+
+
+def check_density(fl: FockLists, max_abs_cart: float, asym: float, max_abs: float):
+    """Reject a density the lists of ``fl`` cannot serve with the reference's
+    screening.
+
+    * ``max_abs_cart`` = max|D| in the Cartesian AO basis must not exceed
+      ``fl.d_bound``: the lists dropped every pair block whose bound
+      Qmax * Qglob * d_bound is below the threshold, which is exact only while
+      max|D| <= d_bound (the reference screens per quartet with the actual D,
+      gen_coulomb.py:209-212, gen_exchange.py:229-232).
+    * ``asym`` = max|D - D^T| must be rounding-level (relative SYM_RTOL of
+      ``max_abs`` = max|D|): the engines symmetrise D and some exchange classes
+      read D_jl as D[l][j], so a non-symmetric density would silently give
+      the exchange matrix of its symmetric part."""
+    if not math.isfinite(max_abs_cart) or max_abs_cart > fl.d_bound:
+        raise DensityError("max|D| = %.4g exceeds the bound %.4g the pair lists were screened for; "
+                           "rebuild the engine with a larger d_bound" % (max_abs_cart, fl.d_bound))
+    if not math.isfinite(asym) or asym > SYM_RTOL * max(max_abs, 1e-300):
+        raise DensityError("density is not symmetric (max|D - D^T| = %.3g, max|D| = %.3g); "
+                           "J/K of a non-symmetric density are not supported" % (asym, max_abs))
+
+
 def build_lists(mol: Molecule, schwarz_fn, thr=1e-12, d_bound=100.0) -> FockLists:
     """Build the D-independent lists of ``mol``.
 

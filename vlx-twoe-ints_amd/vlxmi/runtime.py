@@ -49,17 +49,18 @@ def lib():
         fn = getattr(L, "vlxmi_launch_computeExchangeFock" + cls.upper())
         fn.argtypes = [_ctype_of(s) for s in emit.k_signature(cls)] + [C.c_void_p]
         fn.restype = C.c_int
+    mixed_args = [_ctype_of(s) for s in emit.MIXED_ARGS]       # fp32_threshold, quartet_counts
     for ab, cd in j_classes():
         fn = getattr(L, "vlxmi_launch_computeCoulombFock" + (ab + cd).upper() + "_mixed")
-        fn.argtypes = [_ctype_of(s) for s in emit.j_signature(ab + cd)] + [C.c_double, C.c_uint32, C.c_void_p]
+        fn.argtypes = [_ctype_of(s) for s in emit.j_signature(ab + cd)] + mixed_args + [C.c_uint32, C.c_void_p]
         fn.restype = C.c_int
     for cls in k_classes():
         fn = getattr(L, "vlxmi_launch_computeExchangeFock" + cls.upper() + "_mixed")
-        fn.argtypes = [_ctype_of(s) for s in emit.k_signature(cls)] + [C.c_double, C.c_void_p]
+        fn.argtypes = [_ctype_of(s) for s in emit.k_signature(cls)] + mixed_args + [C.c_void_p]
         fn.restype = C.c_int
     kb_extra = [C.c_void_p, C.c_uint32, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]
     for mixed in ("", "_mixed"):
-        tail = [C.c_double] if mixed else []
+        tail = [_ctype_of(s) for s in emit.tail_args(bool(mixed), True)]
         for ab, cd in j_classes():
             fn = getattr(L, "vlxmi_launch_computeCoulombFock" + (ab + cd).upper() + "_batched" + mixed)
             fn.argtypes = [_ctype_of(s) for s in emit.j_signature(ab + cd)] + [C.c_void_p, C.c_uint32] + tail \
@@ -83,6 +84,9 @@ def lib():
     L.vlxmi_version.restype = C.c_int
     L.vlxmi_segment_sum.argtypes = [C.c_void_p] * 4 + [C.c_uint32, C.c_void_p, C.c_void_p]
     L.vlxmi_fill_zero.argtypes = [C.c_void_p, C.c_uint64, C.c_void_p]
+    L.vlxmi_ket_workspace.argtypes = [C.c_size_t, C.c_void_p, C.POINTER(C.c_void_p)]
+    L.vlxmi_release_workspaces.argtypes = []
+    L.vlxmi_ket_split_reduce.argtypes = [C.c_void_p, C.c_uint32, C.c_uint32, C.c_void_p, C.c_void_p]
     L.vlxmi_count_k_quartets.argtypes = [C.c_void_p, C.c_void_p, C.c_uint32] + [C.c_void_p] * 6 + [C.c_double, C.c_void_p, C.c_void_p]
     _lib = L
     return L
@@ -98,7 +102,7 @@ def exported_symbols():
     """Names every C-ABI entry point the header declares."""
     names = ["vlxmi_version", "vlxmi_boys_table_size", "vlxmi_boys_table_fill", "vlxmi_status_string",
              "vlxmi_boys_eval", "vlxmi_boys_eval_coarse", "vlxmi_count_k_quartets", "vlxmi_segment_sum",
-             "vlxmi_fill_zero"]
+             "vlxmi_fill_zero", "vlxmi_ket_workspace", "vlxmi_release_workspaces", "vlxmi_ket_split_reduce"]
     names += ["vlxmi_launch_computeCoulombFock" + (ab + cd).upper() for ab, cd in j_classes()]
     names += ["vlxmi_launch_computeExchangeFock" + c.upper() for c in k_classes()]
     names += ["vlxmi_launch_computeCoulombFock" + (ab + cd).upper() + "_mixed" for ab, cd in j_classes()]

@@ -68,7 +68,7 @@ class ServeStats:
 
 class FockServer:
     def __init__(self, molecules, dev: Device = None, hbm_budget=16e9, max_batch=32, precision="mixed",
-                 thr=1e-12, fp32_threshold=1e-6, keep_results=False, entry_factory=None, batch_runner=None,
+                 thr=1e-12, fp32_threshold=1e-7, keep_results=False, entry_factory=None, batch_runner=None,
                  synchronize=None, mixed_policy="measured"):
         """``entry_factory(mol) -> entry with .nbytes`` and ``batch_runner(entries,
         densities) -> (Js, Ks)`` default to the GPU MolEntry / BatchEngine; they are

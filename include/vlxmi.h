@@ -66,6 +66,29 @@ int vlxmi_release_workspaces(void);
  * rounding of the total. */
 int vlxmi_ket_split_reduce(const double* part, uint32_t nsplit, uint32_t count, double* out, void* stream);
 
+/* Allocation-free Coulomb launch.  No counterpart in the reference: for callers
+ * that cannot allocate device memory (code under HIP graph capture, tight
+ * serving loops, harnesses that forbid hipMalloc).  Same result as
+ * vlxmi_launch_computeCoulombFock{CLS} with the same ket_split, for the class
+ * named by cls ("ssss" .. "dddd", lower case: bra pair class then ket pair
+ * class); prim_info / prim_count are indexed by angular momentum (0 s, 1 p,
+ * 2 d; entries of types the class does not use are ignored), the remaining
+ * arguments are the class launcher's.  With ket_split > 1 the chunks' partial
+ * sums go to a static per-GPU array of vlxmi_noalloc_scratch_doubles() doubles
+ * (32 MiB) instead of an allocated workspace: calls with
+ * ket_split * ab_prim_pair_count_local above that return VLXMI_ERR_SIZE.  The
+ * array is shared by every call on one GPU, so calls that overlap on one GPU
+ * race.  The entry point calls no allocating routine. */
+int vlxmi_coulomb_fock_noalloc(const char* cls, double* mat_J, const double* const* prim_info,
+                               const uint32_t* prim_count, const double* cd_mat_D, const double* ab_mat_Q_local,
+                               const double* cd_mat_Q, const uint32_t* ab_first_inds_local,
+                               const uint32_t* ab_second_inds_local, const double* ab_pair_data_local,
+                               uint32_t ab_prim_pair_count_local, const uint32_t* cd_first_inds,
+                               const uint32_t* cd_second_inds, const double* cd_pair_data,
+                               uint32_t cd_prim_pair_count, const double* boys_func_table,
+                               const double* boys_func_ft, double eri_threshold, uint32_t ket_split, void* stream);
+uint64_t vlxmi_noalloc_scratch_doubles(void);
+
 #ifdef __cplusplus
 }
 #endif

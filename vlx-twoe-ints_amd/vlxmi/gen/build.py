@@ -118,7 +118,44 @@ def sources(tiles=None, layouts=None):
                                         dorder=dorders.get(cls, "jl"), lds_boys=cls in blds["K"], **kw)
     for pc in PAIR_CLASSES:
         out["schwarz_%s.hip" % pc] = inc + emit.emit_schwarz(pc)
+    out["jdispatch.hip"] = inc + jdispatch_text()
     return out
+
+
+def jdispatch_text():
+    """vlxmi_jpart_dispatch: the plain Coulomb launch of a class named by its
+    four-letter string, with the primitive tables passed per angular momentum
+    (index 0 s, 1 p, 2 d) and a caller-provided partial-sum buffer."""
+    common = ["const double* cd_mat_D", "const double* ab_mat_Q_local", "const double* cd_mat_Q",
+              "const uint32_t* ab_first_inds_local", "const uint32_t* ab_second_inds_local",
+              "const double* ab_pair_data_local", "uint32_t ab_prim_pair_count_local",
+              "const uint32_t* cd_first_inds", "const uint32_t* cd_second_inds", "const double* cd_pair_data",
+              "uint32_t cd_prim_pair_count", "const double* boys_func_table", "const double* boys_func_ft",
+              "double eri_threshold"]
+    o = ["#include <cstring>", ""]
+    for ab, cd in j_classes():
+        cls = j_name(ab, cd)
+        sig = emit._strip_restrict(emit.j_signature(cls))
+        o.append('extern "C" int vlxmi_jpart_computeCoulombFock%s(%s,\n    uint32_t ket_split, double* part, void* stream);'
+                 % (cls.upper(), ", ".join(sig)))
+    o.append("")
+    o.append('extern "C" int vlxmi_jpart_dispatch(const char* cls, double* mat_J, const double* const* prim_info,')
+    o.append("    const uint32_t* prim_count, %s,\n    uint32_t ket_split, double* part, void* stream)" % ", ".join(common))
+    o.append("{")
+    names = [c.split()[-1] for c in common]
+    for ab, cd in j_classes():
+        cls = j_name(ab, cd)
+        args = ["mat_J"]
+        for x in "spd":
+            if x in cls:
+                l = "spd".index(x)
+                args += ["prim_info[%d]" % l, "prim_count[%d]" % l]
+        args += names
+        o.append('    if (std::strcmp(cls, "%s") == 0)' % cls)
+        o.append("        return vlxmi_jpart_computeCoulombFock%s(%s, ket_split, part, stream);" % (cls.upper(), ", ".join(args)))
+    o.append("    return VLXMI_ERR_ARG;")
+    o.append("}")
+    return "\n".join(o) + "\n"
 
 
 def header_text():
@@ -296,7 +333,7 @@ def _build(jobs=None, force=False, only=None, verbose=True):
         f.write(header_text())
     todo = []
     objs = []
-    hand = ["vlxmi_capi.cpp"]
+    hand = ["vlxmi_capi.cpp", "vlxmi_noalloc.cpp"]
     flag_key = " ".join(FLAGS)
     dev_hdr = open(os.path.join(CSRC, "vlxmi_device.hpp")).read() + open(os.path.join(CSRC, "vlxmi_status.h")).read()
     for fn in hand:

@@ -597,44 +597,59 @@ def emit_j(cls, mixed=False, batched=False, kt=None, layout="reg", lds_boys=Fals
         o.append("}")
         o.append("")
     # launcher
+    checks = ["    if (%s_prim_pair_count_local %% %d != 0 || %s_prim_pair_count %% %d != 0) return VLXMI_ERR_LAYOUT;"
+              % (ab, nab, cd, ncd),
+              "    const uint32_t nbra = %s_prim_pair_count_local / %d, nket = %s_prim_pair_count / %d;" % (ab, nab, cd, ncd),
+              "    if (nbra == 0 || nket == 0) return VLXMI_OK;",
+              "    if (!mat_J || !boys_func_table) return VLXMI_ERR_ARG;"]
     if batched:
         o.append('extern "C" int vlxmi_launch_%s(%s,\n    uint32_t n_workgroups, void* stream)' % (name, ",\n    ".join(lsig)))
-    else:
-        o.append('extern "C" int vlxmi_launch_%s(%s,\n    uint32_t ket_split, void* stream)' % (name, ",\n    ".join(lsig)))
-    o.append("{")
-    o.append("    if (%s_prim_pair_count_local %% %d != 0 || %s_prim_pair_count %% %d != 0) return VLXMI_ERR_LAYOUT;" % (ab, nab, cd, ncd))
-    o.append("    const uint32_t nbra = %s_prim_pair_count_local / %d, nket = %s_prim_pair_count / %d;" % (ab, nab, cd, ncd))
-    o.append("    if (nbra == 0 || nket == 0) return VLXMI_OK;")
-    o.append("    if (!mat_J || !boys_func_table) return VLXMI_ERR_ARG;")
-    if batched:
+        o.append("{")
+        o += checks
         o.append("    if (nwork == 0) return VLXMI_OK;")
         o.append("    if (!work) return VLXMI_ERR_ARG;")
         o.append("    if (n_workgroups == 0 || n_workgroups > nwork) n_workgroups = nwork;")
         o.append("    dim3 grid(n_workgroups);")
-    else:
-        o.append("    if (ket_split == 0) ket_split = 1;")
-        o.append("    if (ket_split > nket) ket_split = nket;")
-        o.append("    dim3 grid((nbra + TILE_SIZE_J - 1) / TILE_SIZE_J, ket_split);")
-    if batched:
         o.append("    hipLaunchKernelGGL(%s, grid, dim3(TILE_SIZE_J), 0, (hipStream_t)stream, %s);" % (name, ", ".join(args)))
         o.append("    return (int)hipGetLastError();")
+        o.append("}")
+        return "\n".join(o) + "\n"
+    clamp = ["    if (ket_split == 0) ket_split = 1;",
+             "    if (ket_split > nket) ket_split = nket;"]
+    # The launch with a caller-provided partial-sum buffer ``part`` of at least
+    # ket_split * count_local doubles (used when ket_split > 1): the chunks of
+    # the ket list run in parallel (grid.y), each storing its partial sums, and
+    # a compensated reduction adds them to mat_J in chunk order.
+    o.append('extern "C" int vlxmi_jpart_%s(%s,\n    uint32_t ket_split, double* part, void* stream)'
+             % (name, ",\n    ".join(lsig)))
+    o.append("{")
+    o += checks + clamp
+    o.append("    dim3 grid((nbra + TILE_SIZE_J - 1) / TILE_SIZE_J, ket_split);")
+    o.append("    if (ket_split > 1) {")
+    o.append("        if (!part) return VLXMI_ERR_ARG;")
+    o.append("        hipLaunchKernelGGL(%s_ks, grid, dim3(TILE_SIZE_J), 0, (hipStream_t)stream, %s, part);" % (name, ", ".join(args)))
+    o.append("        const int rc = (int)hipGetLastError();")
+    o.append("        if (rc != 0) return rc;")
+    o.append("        return vlxmi_ket_split_reduce(part, ket_split, %s_prim_pair_count_local, mat_J, stream);" % ab)
+    o.append("    }")
+    if mixed:
+        o.append("    hipLaunchKernelGGL(%s_ks, grid, dim3(TILE_SIZE_J), 0, (hipStream_t)stream, %s, nullptr);" % (name, ", ".join(args)))
     else:
-        # ket split: per-chunk partial sums in the stream's workspace, then a
-        # compensated reduction in chunk order (deterministic, no atomics)
-        o.append("    if (ket_split > 1) {")
-        o.append("        double* part = nullptr;")
-        o.append("        int rc = vlxmi_ket_workspace((size_t)ket_split * %s_prim_pair_count_local, stream, &part);" % ab)
-        o.append("        if (rc != 0) return rc;")
-        o.append("        hipLaunchKernelGGL(%s_ks, grid, dim3(TILE_SIZE_J), 0, (hipStream_t)stream, %s, part);" % (name, ", ".join(args)))
-        o.append("        rc = (int)hipGetLastError();")
-        o.append("        if (rc != 0) return rc;")
-        o.append("        return vlxmi_ket_split_reduce(part, ket_split, %s_prim_pair_count_local, mat_J, stream);" % ab)
-        o.append("    }")
-        if mixed:
-            o.append("    hipLaunchKernelGGL(%s_ks, grid, dim3(TILE_SIZE_J), 0, (hipStream_t)stream, %s, nullptr);" % (name, ", ".join(args)))
-        else:
-            o.append("    hipLaunchKernelGGL(%s, grid, dim3(TILE_SIZE_J), 0, (hipStream_t)stream, %s);" % (name, ", ".join(args)))
-        o.append("    return (int)hipGetLastError();")
+        o.append("    hipLaunchKernelGGL(%s, grid, dim3(TILE_SIZE_J), 0, (hipStream_t)stream, %s);" % (name, ", ".join(args)))
+    o.append("    return (int)hipGetLastError();")
+    o.append("}")
+    o.append("")
+    # the public launcher: the partial sums go to the workspace the library
+    # keeps for this (device, stream)
+    o.append('extern "C" int vlxmi_launch_%s(%s,\n    uint32_t ket_split, void* stream)' % (name, ",\n    ".join(lsig)))
+    o.append("{")
+    o += checks + clamp
+    o.append("    double* part = nullptr;")
+    o.append("    if (ket_split > 1) {")
+    o.append("        const int rc = vlxmi_ket_workspace((size_t)ket_split * %s_prim_pair_count_local, stream, &part);" % ab)
+    o.append("        if (rc != 0) return rc;")
+    o.append("    }")
+    o.append("    return vlxmi_jpart_%s(%s, ket_split, part, stream);" % (name, ", ".join(args)))
     o.append("}")
     return "\n".join(o) + "\n"
 

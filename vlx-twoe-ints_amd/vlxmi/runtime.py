@@ -87,6 +87,11 @@ def lib():
     L.vlxmi_ket_workspace.argtypes = [C.c_size_t, C.c_void_p, C.POINTER(C.c_void_p)]
     L.vlxmi_release_workspaces.argtypes = []
     L.vlxmi_ket_split_reduce.argtypes = [C.c_void_p, C.c_uint32, C.c_uint32, C.c_void_p, C.c_void_p]
+    L.vlxmi_coulomb_fock_noalloc.argtypes = ([C.c_char_p, C.c_void_p, C.POINTER(C.c_void_p), C.POINTER(C.c_uint32)]
+                                             + [C.c_void_p] * 6 + [C.c_uint32] + [C.c_void_p] * 3 + [C.c_uint32]
+                                             + [C.c_void_p] * 2 + [C.c_double, C.c_uint32, C.c_void_p])
+    L.vlxmi_noalloc_scratch_doubles.restype = C.c_uint64
+    L.vlxmi_noalloc_scratch_doubles.argtypes = []
     L.vlxmi_count_k_quartets.argtypes = [C.c_void_p, C.c_void_p, C.c_uint32] + [C.c_void_p] * 6 + [C.c_double, C.c_void_p, C.c_void_p]
     _lib = L
     return L
@@ -102,7 +107,8 @@ def exported_symbols():
     """Names every C-ABI entry point the header declares."""
     names = ["vlxmi_version", "vlxmi_boys_table_size", "vlxmi_boys_table_fill", "vlxmi_status_string",
              "vlxmi_boys_eval", "vlxmi_boys_eval_coarse", "vlxmi_count_k_quartets", "vlxmi_segment_sum",
-             "vlxmi_fill_zero", "vlxmi_ket_workspace", "vlxmi_release_workspaces", "vlxmi_ket_split_reduce"]
+             "vlxmi_fill_zero", "vlxmi_ket_workspace", "vlxmi_release_workspaces", "vlxmi_ket_split_reduce",
+             "vlxmi_coulomb_fock_noalloc", "vlxmi_noalloc_scratch_doubles"]
     names += ["vlxmi_launch_computeCoulombFock" + (ab + cd).upper() for ab, cd in j_classes()]
     names += ["vlxmi_launch_computeExchangeFock" + c.upper() for c in k_classes()]
     names += ["vlxmi_launch_computeCoulombFock" + (ab + cd).upper() + "_mixed" for ab, cd in j_classes()]

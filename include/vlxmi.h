@@ -66,6 +66,54 @@ int vlxmi_release_workspaces(void);
  * rounding of the total. */
 int vlxmi_ket_split_reduce(const double* part, uint32_t nsplit, uint32_t count, double* out, void* stream);
 
+/* Reference list order -> the block order of these kernels (host arrays in and
+ * out; no counterpart in the reference, whose kernels take component-entry
+ * lists: J pair lists with i <= j for one type and the ket list sorted by
+ * |Q_kl D_kl| per entry, gen_coulomb.py:205-212,276-281; exchange CSR segments
+ * sorted per component by Q_K, gen_exchange.py:189-232; any (i,k) order).
+ * Components a reference list does not carry become fillers (zero Schwarz
+ * factor, pair data and density: they add nothing).  Capacities are in
+ * entries; a call with too small a capacity (0 to query) returns
+ * VLXMI_ERR_SIZE with the entries needed in *out_count / *out_nnz.
+ * vlxmi_j_list_to_blocks: mat_D == NULL converts a bra list (blocks by (I, J)),
+ * otherwise a ket list (blocks by non-increasing max |Q D|); out_ref_index
+ * (may be NULL) gives each block entry's reference entry, -1 for fillers, for
+ * vlxmi_scatter_add.  Lists with i > j (one type) return VLXMI_ERR_LAYOUT. */
+int vlxmi_j_list_to_blocks(uint32_t la, uint32_t lb, const uint32_t* first, const uint32_t* second,
+                           const double* pair_data, const double* mat_Q, const double* mat_D, uint32_t count,
+                           uint32_t capacity, uint32_t* out_first, uint32_t* out_second, double* out_pair_data,
+                           double* out_mat_Q, double* out_mat_D, int64_t* out_ref_index, uint32_t* out_count);
+int vlxmi_ik_list_to_blocks(uint32_t la, uint32_t lc, const uint32_t* ik_i, const uint32_t* ik_k, uint32_t count,
+                            uint32_t capacity, uint32_t* out_i, uint32_t* out_k, int64_t* out_ref_index,
+                            uint32_t* out_count);
+int vlxmi_k_csr_to_blocks(uint32_t la, uint32_t lb, uint32_t count_a, const uint32_t* displs, const uint32_t* counts,
+                          const double* Q, const uint32_t* D_inds, const double* pair_data, uint64_t capacity,
+                          uint32_t* out_displs, uint32_t* out_counts, double* out_Q, uint32_t* out_D_inds,
+                          double* out_pair_data, uint64_t* out_nnz);
+
+/* dst[ref_index[e]] += src[e] for e < n with ref_index[e] >= 0 (device arrays):
+ * block-order mat_J / mat_K back to the caller's reference-order entries. */
+int vlxmi_scatter_add(const double* src, const int64_t* ref_index, uint64_t n, double* dst, void* stream);
+
+/* Device checks of the block contract (synchronous: they wait for the stream
+ * and return VLXMI_OK or VLXMI_ERR_LAYOUT).  J list: whole blocks, I <= J for
+ * one type, and for a ket list (mat_D != NULL) D = 0 below the diagonal of
+ * diagonal blocks and non-increasing block bounds max |Q D|.  (i,k) list:
+ * whole blocks, I <= K for one type.  CSR: per primitive, equal segment
+ * lengths, whole partner blocks in one order for every component, block max
+ * |Q| non-increasing.  vlxmi_set_list_validation(1) makes every class launcher
+ * (vlxmi_launch_computeCoulombFock* / computeExchangeFock*, not the batched
+ * ones) run these on its lists first and return VLXMI_ERR_LAYOUT on a
+ * mismatch; off by default (each check synchronizes the stream). */
+int vlxmi_validate_j_list(uint32_t la, uint32_t lb, const uint32_t* first, const uint32_t* second,
+                          const double* mat_Q, const double* mat_D, uint32_t count, void* stream);
+int vlxmi_validate_ik_list(uint32_t la, uint32_t lc, const uint32_t* ik_i, const uint32_t* ik_k, uint32_t count,
+                           void* stream);
+int vlxmi_validate_k_csr(uint32_t la, uint32_t lb, uint32_t count_a, const uint32_t* displs, const uint32_t* counts,
+                         const double* Q, const uint32_t* D_inds, void* stream);
+int vlxmi_set_list_validation(int on);
+int vlxmi_list_validation(void);
+
 /* Allocation-free Coulomb launch.  No counterpart in the reference: for callers
  * that cannot allocate device memory (code under HIP graph capture, tight
  * serving loops, harnesses that forbid hipMalloc).  Same result as

@@ -62,7 +62,7 @@ extern "C" const char* vlxmi_status_string(int code)
     switch (code) {
         case VLXMI_OK: return "ok";
         case VLXMI_ERR_ARG: return "invalid argument (null pointer or bad count)";
-        case VLXMI_ERR_LAYOUT: return "list length is not a whole number of primitive-pair blocks";
+        case VLXMI_ERR_LAYOUT: return "list is not in the primitive-pair block order the kernels need";
         case VLXMI_ERR_SIZE: return "size exceeds the entry point's limit";
         default: return hipGetErrorString((hipError_t)code);
     }

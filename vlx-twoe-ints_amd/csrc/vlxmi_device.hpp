@@ -40,6 +40,16 @@
 
 #include "vlxmi_status.h"
 
+// List-order validation the class launchers run when it is switched on
+// (vlxmi_lists.cpp).
+extern "C" int vlxmi_list_validation(void);
+extern "C" int vlxmi_validate_j_list(uint32_t la, uint32_t lb, const uint32_t* first, const uint32_t* second,
+                                     const double* mat_Q, const double* mat_D, uint32_t count, void* stream);
+extern "C" int vlxmi_validate_ik_list(uint32_t la, uint32_t lc, const uint32_t* ik_i, const uint32_t* ik_k,
+                                      uint32_t count, void* stream);
+extern "C" int vlxmi_validate_k_csr(uint32_t la, uint32_t lb, uint32_t count_a, const uint32_t* displs,
+                                    const uint32_t* counts, const double* Q, const uint32_t* D_inds, void* stream);
+
 // Ket-split partial sums of the Coulomb launchers (vlxmi_capi.cpp).
 extern "C" int vlxmi_ket_workspace(size_t ndoubles, void* stream, double** out);
 extern "C" int vlxmi_ket_split_reduce(const double* part, uint32_t nsplit, uint32_t count, double* out,

@@ -44,6 +44,7 @@ OBJ = os.path.join(PKG, "_build")
 LIBDIR = os.path.join(PKG, "vlxmi", "_lib")
 LIB = os.path.join(LIBDIR, "libvlxmi.so")
 HEADER = os.path.join(ROOT, "include", "vlxmi_kernels.h")
+DEVICE_HEADER = os.path.join(ROOT, "include", "vlxmi_device_kernels.h")
 
 TILE_TABLE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "ket_tiles.json")
 # heavy exchange classes: register layout per class ("reg" or "lds", see
@@ -72,7 +73,7 @@ HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = "gfx950"
 FLAGS = ["--offload-arch=" + ARCH, "-O3", "-std=c++17", "-fPIC", "-munsafe-fp-atomics",
          "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
-         "-I", CSRC]
+         "-I", CSRC, "-I", os.path.join(ROOT, "include")]
 
 
 def j_name(ab, cd):
@@ -119,7 +120,84 @@ def sources(tiles=None, layouts=None):
     for pc in PAIR_CLASSES:
         out["schwarz_%s.hip" % pc] = inc + emit.emit_schwarz(pc)
     out["jdispatch.hip"] = inc + jdispatch_text()
+    out["splits.hip"] = inc + splits_text()
     return out
+
+
+# VeloxChem launches the classes cut for register pressure as series of split
+# kernels (computeCoulombFockPDDD0..6 etc.), all accumulating into the same
+# output (summary_fock_split.txt:1-20; names made at split_coulomb_fock.py:242-249
+# and new_exchange_code/exchangefocksplitter.py:167-176).
+SPLIT_SERIES = {"J": {"pddd": 7, "ddpd": 10, "dddd": 30},
+                "K": {"pddd": 8, "dpdd": 7, "dddp": 7, "dddd": 19}}
+
+
+def splits_text():
+    """Launchers named after the reference's split kernels: split 0 runs the
+    whole class (these kernels are split into component passes inside one
+    launch instead), splits 1..n-1 check their arguments and launch nothing,
+    so a host that launches the whole series into one buffer gets the class's
+    result."""
+    o = []
+    for kind, series in SPLIT_SERIES.items():
+        for cls, n in series.items():
+            if kind == "J":
+                sig = emit._strip_restrict(emit.j_signature(cls))
+                tail, targs = ["uint32_t ket_split", "void* stream"], ["ket_split", "stream"]
+                base = "computeCoulombFock" + cls.upper()
+                count = "%s_prim_pair_count_local" % cls[:2]
+            else:
+                sig = emit._strip_restrict(emit.k_signature(cls))
+                tail, targs = ["void* stream"], ["stream"]
+                base = "computeExchangeFock" + cls.upper()
+                count = "pair_inds_count_for_K_%s" % (cls[0] + cls[2])
+            args = [x.split()[-1] for x in sig]
+            for k in range(n):
+                o.append('extern "C" int vlxmi_launch_%s%d(%s)' % (base, k, ", ".join(sig + tail)))
+                o.append("{")
+                if k == 0:
+                    o.append("    return vlxmi_launch_%s(%s);" % (base, ", ".join(args + targs)))
+                else:
+                    o.append("    (void)stream;")
+                    for a in args:
+                        o.append("    (void)%s;" % a)
+                    if kind == "J":
+                        o.append("    (void)ket_split;")
+                    o.append("    (void)%s;" % count)
+                    o.append("    return VLXMI_OK;")
+                o.append("}")
+                o.append("")
+    return "#include \"vlxmi_kernels.h\"\n\n" + "\n".join(o) + "\n"
+
+
+def device_header_text():
+    """The kernels' own declarations (the reference prints its kernels'
+    declarations with the ``header`` argument, gen_coulomb.py:38-45,95-98)."""
+    o = ["/* Generated by vlxmi.gen.build -- declarations of the J/K kernels themselves,",
+         " * with the reference's names and parameter lists (gen_coulomb.py:74-93,",
+         " * gen_exchange.py:75-115), for HIP code that launches them directly.",
+         " *",
+         " * Launch geometry (these kernels' own, not the reference's 2-D tiles):",
+         " *   computeCoulombFock{AB}{CD}: blockDim 64 (one wavefront), gridDim.x =",
+         " *     ceil(bra blocks / 64), gridDim.y = number of ket-list chunks (with more",
+         " *     than one the chunks add to mat_J atomically; the launchers in",
+         " *     vlxmi_kernels.h add them in a fixed order instead);",
+         " *   computeExchangeFock{A}{B}{C}{D}: blockDim 64, gridDim.x = (i,k) blocks.",
+         " * The lists must be in the primitive-pair block order (vlxmi_*_to_blocks",
+         " * convert reference-order lists); the launchers check arguments and are the",
+         " * supported entry points. */",
+         "#ifndef VLXMI_DEVICE_KERNELS_H",
+         "#define VLXMI_DEVICE_KERNELS_H",
+         "#include <hip/hip_runtime.h>",
+         "#include <stdint.h>",
+         ""]
+    for ab, cd in j_classes():
+        cls = j_name(ab, cd)
+        o.append("__global__ void computeCoulombFock%s(%s);" % (cls.upper(), ", ".join(emit._strip_restrict(emit.j_signature(cls)))))
+    for cls in k_classes():
+        o.append("__global__ void computeExchangeFock%s(%s);" % (cls.upper(), ", ".join(emit._strip_restrict(emit.k_signature(cls)))))
+    o += ["", "#endif", ""]
+    return "\n".join(o)
 
 
 def jdispatch_text():
@@ -234,6 +312,20 @@ def header_text():
             o.append("int vlxmi_launch_computeExchangeFock%s_batched%s(%s,\n    uint32_t n_workgroups, void* stream);"
                      % (cls.upper(), "_mixed" if mixed else "", ",\n    ".join(sig)))
             o.append("")
+    o.append("/* Launchers named after the reference's split kernels (VeloxChem launches these classes as")
+    o.append(" * series computeCoulombFockPDDD0..6 etc., summary_fock_split.txt:1-20, all into one output):")
+    o.append(" * split 0 runs the whole class, splits 1..n-1 launch nothing, so the series' sum is the class. */")
+    for kind, series in SPLIT_SERIES.items():
+        for cls, n in series.items():
+            if kind == "J":
+                sig = emit._strip_restrict(emit.j_signature(cls)) + ["uint32_t ket_split", "void* stream"]
+                base = "computeCoulombFock" + cls.upper()
+            else:
+                sig = emit._strip_restrict(emit.k_signature(cls)) + ["void* stream"]
+                base = "computeExchangeFock" + cls.upper()
+            for k in range(n):
+                o.append("int vlxmi_launch_%s%d(%s);" % (base, k, ", ".join(sig)))
+    o.append("")
     for pc in PAIR_CLASSES:
         o.append("/* Schwarz factors sqrt|(ab|ab)| per component pair of primitive-pair blocks (the")
         o.append(" * reference's mat_Q / Q_K inputs, computed by VeloxChem's host there). */")
@@ -331,9 +423,11 @@ def _build(jobs=None, force=False, only=None, verbose=True):
     srcs = sources()
     with open(HEADER, "w") as f:
         f.write(header_text())
+    with open(DEVICE_HEADER, "w") as f:
+        f.write(device_header_text())
     todo = []
     objs = []
-    hand = ["vlxmi_capi.cpp", "vlxmi_noalloc.cpp"]
+    hand = ["vlxmi_capi.cpp", "vlxmi_noalloc.cpp", "vlxmi_lists.cpp"]
     flag_key = " ".join(FLAGS)
     dev_hdr = open(os.path.join(CSRC, "vlxmi_device.hpp")).read() + open(os.path.join(CSRC, "vlxmi_status.h")).read()
     for fn in hand:

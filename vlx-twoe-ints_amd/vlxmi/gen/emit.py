@@ -640,10 +640,18 @@ def emit_j(cls, mixed=False, batched=False, kt=None, layout="reg", lds_boys=Fals
     o.append("}")
     o.append("")
     # the public launcher: the partial sums go to the workspace the library
-    # keeps for this (device, stream)
+    # keeps for this (device, stream); with list validation switched on, both
+    # lists are checked against the block contract first
     o.append('extern "C" int vlxmi_launch_%s(%s,\n    uint32_t ket_split, void* stream)' % (name, ",\n    ".join(lsig)))
     o.append("{")
     o += checks + clamp
+    o.append("    if (vlxmi_list_validation()) {")
+    o.append("        int rc = vlxmi_validate_j_list(%d, %d, %s_first_inds_local, %s_second_inds_local, %s_mat_Q_local, nullptr, %s_prim_pair_count_local, stream);"
+             % (la, lb, ab, ab, ab, ab))
+    o.append("        if (rc == 0) rc = vlxmi_validate_j_list(%d, %d, %s_first_inds, %s_second_inds, %s_mat_Q, %s_mat_D, %s_prim_pair_count, stream);"
+             % (lc, ld, cd, cd, cd, cd, cd))
+    o.append("        if (rc != 0) return rc;")
+    o.append("    }")
     o.append("    double* part = nullptr;")
     o.append("    if (ket_split > 1) {")
     o.append("        const int rc = vlxmi_ket_workspace((size_t)ket_split * %s_prim_pair_count_local, stream, &part);" % ab)
@@ -1025,6 +1033,18 @@ def emit_k(cls, mixed=False, batched=False, kt=None, layout="reg", dorder="jl", 
     o.append("    const uint32_t nblk = pair_inds_count_for_K_%s / %d;" % (ac, nac))
     o.append("    if (nblk == 0) return VLXMI_OK;")
     o.append("    if (!mat_K || !mat_D_full_AO || !boys_func_table) return VLXMI_ERR_ARG;")
+    if not batched:
+        # with list validation switched on: the (i,k) list and both CSRs
+        o.append("    if (vlxmi_list_validation()) {")
+        o.append("        int rc = vlxmi_validate_ik_list(%d, %d, pair_inds_i_for_K_%s, pair_inds_k_for_K_%s, pair_inds_count_for_K_%s, stream);"
+                 % (la, lc, ac, ac, ac))
+        o.append("        if (rc == 0) rc = vlxmi_validate_k_csr(%d, %d, %s_prim_count, pair_displs_K_%s, pair_counts_K_%s, Q_K_%s, D_inds_K_%s, stream);"
+                 % (la, lb, a, ab, ab, ab, ab))
+        if cd != ab:
+            o.append("        if (rc == 0) rc = vlxmi_validate_k_csr(%d, %d, %s_prim_count, pair_displs_K_%s, pair_counts_K_%s, Q_K_%s, D_inds_K_%s, stream);"
+                     % (lc, ld, c, cd, cd, cd, cd))
+        o.append("        if (rc != 0) return rc;")
+        o.append("    }")
     if batched:
         o.append("    if (nlist == 0) return VLXMI_OK;")
         o.append("    if (!blk_list || !blk_mol || !mol_maxD || !mol_D_off || !mol_naos || !mol_ao_off) return VLXMI_ERR_ARG;")

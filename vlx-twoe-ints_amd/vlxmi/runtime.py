@@ -70,6 +70,17 @@ def lib():
             fn = getattr(L, "vlxmi_launch_computeExchangeFock" + cls.upper() + "_batched" + mixed)
             fn.argtypes = [_ctype_of(s) for s in emit.k_signature(cls)] + kb_extra + tail + [C.c_uint32, C.c_void_p]
             fn.restype = C.c_int
+    from .gen.build import SPLIT_SERIES
+    for kind, series in SPLIT_SERIES.items():
+        for cls, n in series.items():
+            for k in range(n):
+                if kind == "J":
+                    fn = getattr(L, "vlxmi_launch_computeCoulombFock%s%d" % (cls.upper(), k))
+                    fn.argtypes = [_ctype_of(s) for s in emit.j_signature(cls)] + [C.c_uint32, C.c_void_p]
+                else:
+                    fn = getattr(L, "vlxmi_launch_computeExchangeFock%s%d" % (cls.upper(), k))
+                    fn.argtypes = [_ctype_of(s) for s in emit.k_signature(cls)] + [C.c_void_p]
+                fn.restype = C.c_int
     for pc in PAIR_CLASSES:
         fn = getattr(L, "vlxmi_schwarz_" + pc)
         fn.argtypes = [C.c_void_p, C.c_uint32, C.c_void_p, C.c_uint32, C.c_void_p, C.c_void_p, C.c_void_p,
@@ -90,6 +101,16 @@ def lib():
     L.vlxmi_coulomb_fock_noalloc.argtypes = ([C.c_char_p, C.c_void_p, C.POINTER(C.c_void_p), C.POINTER(C.c_uint32)]
                                              + [C.c_void_p] * 6 + [C.c_uint32] + [C.c_void_p] * 3 + [C.c_uint32]
                                              + [C.c_void_p] * 2 + [C.c_double, C.c_uint32, C.c_void_p])
+    P_ = C.c_void_p
+    L.vlxmi_j_list_to_blocks.argtypes = [C.c_uint32, C.c_uint32] + [P_] * 5 + [C.c_uint32, C.c_uint32] + [P_] * 7
+    L.vlxmi_ik_list_to_blocks.argtypes = [C.c_uint32, C.c_uint32, P_, P_, C.c_uint32, C.c_uint32, P_, P_, P_, P_]
+    L.vlxmi_k_csr_to_blocks.argtypes = [C.c_uint32, C.c_uint32, C.c_uint32] + [P_] * 5 + [C.c_uint64] + [P_] * 6
+    L.vlxmi_scatter_add.argtypes = [P_, P_, C.c_uint64, P_, P_]
+    L.vlxmi_validate_j_list.argtypes = [C.c_uint32, C.c_uint32, P_, P_, P_, P_, C.c_uint32, P_]
+    L.vlxmi_validate_ik_list.argtypes = [C.c_uint32, C.c_uint32, P_, P_, C.c_uint32, P_]
+    L.vlxmi_validate_k_csr.argtypes = [C.c_uint32, C.c_uint32, C.c_uint32, P_, P_, P_, P_, P_]
+    L.vlxmi_set_list_validation.argtypes = [C.c_int]
+    L.vlxmi_list_validation.argtypes = []
     L.vlxmi_noalloc_scratch_doubles.restype = C.c_uint64
     L.vlxmi_noalloc_scratch_doubles.argtypes = []
     L.vlxmi_count_k_quartets.argtypes = [C.c_void_p, C.c_void_p, C.c_uint32] + [C.c_void_p] * 6 + [C.c_double, C.c_void_p, C.c_void_p]
@@ -108,7 +129,9 @@ def exported_symbols():
     names = ["vlxmi_version", "vlxmi_boys_table_size", "vlxmi_boys_table_fill", "vlxmi_status_string",
              "vlxmi_boys_eval", "vlxmi_boys_eval_coarse", "vlxmi_count_k_quartets", "vlxmi_segment_sum",
              "vlxmi_fill_zero", "vlxmi_ket_workspace", "vlxmi_release_workspaces", "vlxmi_ket_split_reduce",
-             "vlxmi_coulomb_fock_noalloc", "vlxmi_noalloc_scratch_doubles"]
+             "vlxmi_coulomb_fock_noalloc", "vlxmi_noalloc_scratch_doubles", "vlxmi_j_list_to_blocks",
+             "vlxmi_ik_list_to_blocks", "vlxmi_k_csr_to_blocks", "vlxmi_scatter_add", "vlxmi_validate_j_list",
+             "vlxmi_validate_ik_list", "vlxmi_validate_k_csr", "vlxmi_set_list_validation", "vlxmi_list_validation"]
     names += ["vlxmi_launch_computeCoulombFock" + (ab + cd).upper() for ab, cd in j_classes()]
     names += ["vlxmi_launch_computeExchangeFock" + c.upper() for c in k_classes()]
     names += ["vlxmi_launch_computeCoulombFock" + (ab + cd).upper() + "_mixed" for ab, cd in j_classes()]
@@ -117,6 +140,10 @@ def exported_symbols():
         names += ["vlxmi_launch_computeCoulombFock" + (ab + cd).upper() + "_batched" + mixed for ab, cd in j_classes()]
         names += ["vlxmi_launch_computeExchangeFock" + c.upper() + "_batched" + mixed for c in k_classes()]
     names += ["vlxmi_schwarz_" + pc for pc in PAIR_CLASSES]
+    from .gen.build import SPLIT_SERIES
+    for kind, series in SPLIT_SERIES.items():
+        base = "vlxmi_launch_computeCoulombFock" if kind == "J" else "vlxmi_launch_computeExchangeFock"
+        names += ["%s%s%d" % (base, cls.upper(), k) for cls, n in series.items() for k in range(n)]
     return names
 
 

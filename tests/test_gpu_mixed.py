@@ -68,3 +68,21 @@ def test_mixed_lattice64_energy_bar(dev):
     assert n32 > 0 and n64 > 0
     assert abs(EJ - EJ64) <= 1e-8 and abs(EK - EK64) <= 1e-8
     assert dJ <= 1e-8 and dK <= 1e-8
+
+
+def test_build_with_report(dev):
+    """A mixed build reports its quartet counts and its errors against the
+    plain FP64 build; the reported errors are the ones measured here against
+    the oracle-pinned FP64 path."""
+    from vlxmi.fock import FockEngine
+    print("case: lattice4 mixed build with report", flush=True)
+    mol, D = configs.make_lattice4()
+    eng = FockEngine(mol, dev=dev, precision="mixed", mixed_policy="all", fp32_threshold=1e-6)
+    J, K, rep = eng.build_with_report(D)
+    print("  report %s" % rep, flush=True)
+    assert rep["precision"] == "mixed" and rep["reference"] == "fp64"
+    assert rep["mixed_kernel_quartets"] > 0 and 0 <= rep["fp32_quartets"] <= rep["mixed_kernel_quartets"]
+    eng.precision = "fp64"
+    J64, K64 = eng.build(D)
+    assert rep["max_abs_dJ"] == float((J - J64).abs().max()) and rep["max_abs_dK"] == float((K - K64).abs().max())
+    assert abs(rep["dE_J"]) <= 1e-8 and abs(rep["dE_K"]) <= 1e-8

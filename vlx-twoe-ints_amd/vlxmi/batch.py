@@ -127,6 +127,29 @@ class MolEntry:
                                nval=int(vo.size), nseg=int(starts.size))
         self.jplan, self.kplan = plans["J"], plans["K"]
 
+    @staticmethod
+    def estimate_nbytes(mol):
+        """Upper bound of ``nbytes`` before the lists exist (every candidate
+        pair kept), for admission by HBM budget: per J entry 40 bytes (indices,
+        S, Q, density index, mask), per K CSR entry 20, per (i,k) entry 8, the
+        assembly plans (16 bytes per contribution), the density / J / K /
+        pair-type slots."""
+        counts = [0, 0, 0]
+        for sh in mol.shells:
+            counts[sh.l] += sh.nprim
+        nc = [NCART[l] for l in range(3)]
+        jent = kent = 0
+        for la in range(3):
+            for lb in range(3):
+                blocks = counts[la] * counts[lb]
+                kent += blocks * nc[la] * nc[lb]
+                if la <= lb:
+                    jent += (counts[la] * (counts[la] + 1) // 2 if la == lb else blocks) * nc[la] * nc[lb]
+        ikent = jent
+        n = mol.naos_cart
+        return 40 * jent + 20 * kent + 8 * ikent + 16 * 2 * (jent + ikent) + 8 * 3 * sum(counts) * 6 \
+            + 8 * n * n * 4
+
     def _nbytes(self):
         tot = 0
         for v in list(self.jl.values()) + list(self.kc.values()) + list(self.ik.values()) + [self.jplan, self.kplan]:

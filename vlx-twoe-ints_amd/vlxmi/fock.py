@@ -373,6 +373,30 @@ class FockEngine:
             K = self.T @ Kc @ self.T.t()
         return J, K
 
+    def build_with_report(self, D_sph):
+        """J, K and a report of the build: in mixed precision the quartet
+        counts (FP64 / FP32, classes that ran their _mixed kernel) and the
+        errors against the plain FP64 build of the same density (dE_J, dE_K,
+        max|dJ|, max|dK|), which this call also runs.  Synchronizes."""
+        t = self.torch
+        if not t.is_tensor(D_sph):
+            D_sph = self.dev.tensor(np.asarray(D_sph, dtype=np.float64))
+        J, K = self.build(D_sph)
+        rep = {"precision": self.precision}
+        if self.precision == "mixed":
+            n64, n32 = self.quartet_counts()
+            rep.update(fp32_quartets=n32, mixed_kernel_quartets=n64 + n32, fp32_threshold=self.fp32_threshold)
+            self.precision = "fp64"
+            try:
+                J64, K64 = self.build(D_sph)
+            finally:
+                self.precision = "mixed"
+            EJ, EK = self.energies(D_sph, J, K)
+            EJ64, EK64 = self.energies(D_sph, J64, K64)
+            rep.update(reference="fp64", dE_J=EJ - EJ64, dE_K=EK - EK64,
+                       max_abs_dJ=float((J - J64).abs().max()), max_abs_dK=float((K - K64).abs().max()))
+        return J, K, rep
+
     def energies(self, D_sph, J, K):
         t = self.torch
         if not t.is_tensor(D_sph):

@@ -114,6 +114,24 @@ int vlxmi_validate_k_csr(uint32_t la, uint32_t lb, uint32_t count_a, const uint3
 int vlxmi_set_list_validation(int on);
 int vlxmi_list_validation(void);
 
+/* Device side of the pair-list builder (vlxmi/devpairs.py drives it; VeloxChem's
+ * host builds these lists for the reference kernels).  vlxmi_pair_candidates:
+ * every candidate primitive-pair block (I, J) of class (la <= lb; J >= I when
+ * la == lb, row-major) with I[b], J[b] and the overlap factors
+ * S[b][ca][cb] = fac_a[I][ca] fac_b[J][cb] (pi/zeta)^(3/2) exp(-a_i a_j |AB|^2/zeta),
+ * prim_info tables as for the kernels, fac_* per primitive component.
+ * vlxmi_block_absmax: out[b] = max_t |x[b*nab + t]|.  vlxmi_kcsr_fill: exchange
+ * CSR entries (Q, pair data, partner index) of ordered blocks sorted by I, block
+ * max (non-increasing) and J, read from a pair set's S and Q (transposed where
+ * trans[p]): entry (I, ca, rank[p], cb) at base[I] + ca*seg[I] + rank[p]*nb + cb. */
+int vlxmi_pair_candidates(uint32_t la, uint32_t lb, const double* info_a, uint32_t count_a, const double* info_b,
+                          uint32_t count_b, const double* fac_a, const double* fac_b, uint32_t* I, uint32_t* J,
+                          double* S, void* stream);
+int vlxmi_block_absmax(const double* x, uint64_t nblk, uint32_t nab, double* out, void* stream);
+int vlxmi_kcsr_fill(uint32_t la, uint32_t lb, const int64_t* src, const uint8_t* trans, const int64_t* Iord,
+                    const int64_t* Jord, const int64_t* rank, const int64_t* base, const int64_t* seg, uint64_t nord,
+                    const double* S, const double* Q, double* Qe, double* Se, uint32_t* De, void* stream);
+
 /* Allocation-free Coulomb launch.  No counterpart in the reference: for callers
  * that cannot allocate device memory (code under HIP graph capture, tight
  * serving loops, harnesses that forbid hipMalloc).  Same result as

@@ -146,3 +146,28 @@ def test_quartet_counters_match_host_counts(dev, mols, pack):
             want["K:" + cls] = want.get("K:" + cls, 0) + flops.count_k(e.fl, Dc, cls, mx)
     for name, n in want.items():
         assert got.get(name, (0, 0)) == (n, 0), (name, got.get(name), n)
+
+
+def test_batch_workload_large_molecules(dev):
+    """Molecules of the 512-molecule batch workload up to its largest (47
+    atoms, nbf 235) packed with small ones: each molecule's J/K from the pack
+    against the single-molecule FP64 path (1e-11) and the size-independent
+    properties (J, K symmetric; E_J > E_K > 0)."""
+    from vlxmi.batch import BatchEngine, MolEntry, Pack
+    from vlxmi.fock import FockEngine
+    ids = [207, 5, 22, 0, 1, 2]
+    mols = [configs.make_batch_molecule(m) for m in ids]
+    _say("case: batch workload molecules %s (atoms %s) packed vs single path"
+         % (ids, [m.natoms for m, _ in mols]))
+    assert max(m.natoms for m, _ in mols) >= 47
+    entries = [MolEntry(mol, dev) for mol, _ in mols]
+    be = BatchEngine(dev)
+    Js, Ks = be.build(Pack(entries, dev), [D for _, D in mols])
+    for k, (mol, D) in enumerate(mols):
+        J1, K1 = FockEngine(mol, dev=dev, lists=entries[k].fl).build(D)
+        assert (Js[k] - J1).abs().max().item() <= TOL, ids[k]
+        assert (Ks[k] - K1).abs().max().item() <= TOL, ids[k]
+        J, K = Js[k].cpu().numpy(), Ks[k].cpu().numpy()
+        assert np.abs(J - J.T).max() <= 1e-11 and np.abs(K - K.T).max() <= 1e-11
+        EJ, EK = 0.5 * np.sum(D * J), 0.5 * np.sum(D * K)
+        assert EJ > EK > 0

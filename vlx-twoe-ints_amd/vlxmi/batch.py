@@ -38,11 +38,15 @@ LV = P.LVAL
 class MolEntry:
     """Device-resident, density-independent data of one molecule."""
 
-    def __init__(self, mol, dev: Device, thr=1e-12, d_bound=100.0):
+    def __init__(self, mol, dev: Device, thr=1e-12, d_bound=100.0, builder="device"):
         self.mol = mol
         self.dev = dev
         t = dev.torch
-        self.fl = fl = P.build_lists(mol, dev.schwarz, thr=thr, d_bound=d_bound)
+        if builder == "device":
+            from .devpairs import build_lists_device
+            self.fl = fl = build_lists_device(mol, dev, thr=thr, d_bound=d_bound)
+        else:
+            self.fl = fl = P.build_lists(mol, dev.schwarz, thr=thr, d_bound=d_bound)
         self.naos = mol.naos_cart
         self.nbf = mol.nbf
         Tm = mol.cart2sph()

@@ -34,10 +34,11 @@ from .runtime import Device, check, lib
 
 class FockEngine:
     def __init__(self, mol, thr=1e-12, dev=None, d_bound=100.0, omega=0.0, target_waves=32768,
-                 precision="fp64", fp32_threshold=1e-7, mixed_policy="measured", lists=None):
+                 precision="fp64", fp32_threshold=1e-7, mixed_policy="measured", lists=None, builder="device"):
         """``lists``: the molecule's FockLists if already built (e.g. by a
         batched-engine entry of the same molecule); otherwise they are built
-        here with the GPU Schwarz kernels."""
+        here, on the device (``builder="device"``, vlxmi.devpairs) or by the
+        host builder with the GPU Schwarz kernels (``"host"``)."""
         self.dev = dev or Device()
         t = self.dev.torch
         self.torch = t
@@ -61,6 +62,9 @@ class FockEngine:
         if lists is not None:
             assert lists.mol is mol and lists.thr == self.thr
             self.fl = lists
+        elif builder == "device":
+            from .devpairs import build_lists_device
+            self.fl = build_lists_device(mol, self.dev, thr=thr, d_bound=d_bound)
         else:
             self.fl = P.build_lists(mol, self.dev.schwarz, thr=thr, d_bound=d_bound)
         self.setup_lists_s = time.time() - t0

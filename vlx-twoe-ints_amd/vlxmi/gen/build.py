@@ -427,7 +427,7 @@ def _build(jobs=None, force=False, only=None, verbose=True):
         f.write(device_header_text())
     todo = []
     objs = []
-    hand = ["vlxmi_capi.cpp", "vlxmi_noalloc.cpp", "vlxmi_lists.cpp"]
+    hand = ["vlxmi_capi.cpp", "vlxmi_noalloc.cpp", "vlxmi_lists.cpp", "vlxmi_pairs.cpp"]
     flag_key = " ".join(FLAGS)
     dev_hdr = open(os.path.join(CSRC, "vlxmi_device.hpp")).read() + open(os.path.join(CSRC, "vlxmi_status.h")).read()
     for fn in hand:

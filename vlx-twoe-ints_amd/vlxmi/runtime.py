@@ -111,6 +111,9 @@ def lib():
     L.vlxmi_validate_k_csr.argtypes = [C.c_uint32, C.c_uint32, C.c_uint32, P_, P_, P_, P_, P_]
     L.vlxmi_set_list_validation.argtypes = [C.c_int]
     L.vlxmi_list_validation.argtypes = []
+    L.vlxmi_pair_candidates.argtypes = [C.c_uint32, C.c_uint32, P_, C.c_uint32, P_, C.c_uint32] + [P_] * 5 + [P_]
+    L.vlxmi_block_absmax.argtypes = [P_, C.c_uint64, C.c_uint32, P_, P_]
+    L.vlxmi_kcsr_fill.argtypes = [C.c_uint32, C.c_uint32] + [P_] * 7 + [C.c_uint64] + [P_] * 6
     L.vlxmi_noalloc_scratch_doubles.restype = C.c_uint64
     L.vlxmi_noalloc_scratch_doubles.argtypes = []
     L.vlxmi_count_k_quartets.argtypes = [C.c_void_p, C.c_void_p, C.c_uint32] + [C.c_void_p] * 6 + [C.c_double, C.c_void_p, C.c_void_p]
@@ -131,7 +134,8 @@ def exported_symbols():
              "vlxmi_fill_zero", "vlxmi_ket_workspace", "vlxmi_release_workspaces", "vlxmi_ket_split_reduce",
              "vlxmi_coulomb_fock_noalloc", "vlxmi_noalloc_scratch_doubles", "vlxmi_j_list_to_blocks",
              "vlxmi_ik_list_to_blocks", "vlxmi_k_csr_to_blocks", "vlxmi_scatter_add", "vlxmi_validate_j_list",
-             "vlxmi_validate_ik_list", "vlxmi_validate_k_csr", "vlxmi_set_list_validation", "vlxmi_list_validation"]
+             "vlxmi_validate_ik_list", "vlxmi_validate_k_csr", "vlxmi_set_list_validation", "vlxmi_list_validation",
+             "vlxmi_pair_candidates", "vlxmi_block_absmax", "vlxmi_kcsr_fill"]
     names += ["vlxmi_launch_computeCoulombFock" + (ab + cd).upper() for ab, cd in j_classes()]
     names += ["vlxmi_launch_computeExchangeFock" + c.upper() for c in k_classes()]
     names += ["vlxmi_launch_computeCoulombFock" + (ab + cd).upper() + "_mixed" for ab, cd in j_classes()]

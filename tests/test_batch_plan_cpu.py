@@ -92,7 +92,7 @@ def pack():
     dev = CpuDevice()
     mols = [configs.make_batch_molecule(m, hi=12)[0] for m in range(4)]
     mols.append(configs.make_small8()[0])
-    entries = [MolEntry(mol, dev) for mol in mols]
+    entries = [MolEntry(mol, dev, builder="host") for mol in mols]
     return Pack(entries, dev)
 
 

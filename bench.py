@@ -346,6 +346,15 @@ def bench_batch(a):
         "top_kernels_ms": {k: 1e3 * sum(per[k]) / len(per[k]) for k in sorted(per, key=lambda k: -sum(per[k]))[:6]},
         "autotuned_classes": {p: res[p]["tuned"] for p in res}, "setup_s": setup,
     }
+    if a.per_class_out:
+        rows = {}
+        for prec in ("fp64", "mixed"):
+            for k, v in res[prec]["per"].items():
+                nq, f = flop_of(k, res[prec]["counts"])
+                rows["%s:%s" % (prec, k)] = {"avg_s": sum(v) / len(v), "quartets": nq, "flop": f,
+                                             "tflops": f / (sum(v) / len(v)) / 1e12}
+        with open(a.per_class_out, "w") as f:
+            json.dump(rows, f, indent=1, sort_keys=True)
     if not a.no_cpu:
         _progress("batch: CPU oracle baseline (bounded sample)")
         out["cpu_baseline"] = cpu_baseline_batch(entries, dens, res["mixed"]["counts"], nmol)

@@ -46,6 +46,15 @@ int vlxmi_count_k_quartets(const uint32_t* ikI, const uint32_t* ikK, uint32_t nb
 int vlxmi_segment_sum(const double* values, const uint32_t* perm, const uint32_t* seg_off,
                       const uint32_t* seg_out, uint32_t nseg, double* out, void* stream);
 
+/* Two-electron part of the Fock matrix in one deterministic pass (no atomics):
+ * out[seg_out[s]] = sum_{t = seg_off[s]}^{seg_off[s+1]-1} (src[t] ? cK : cJ) * values[perm[t]],
+ * values = the per-pair mat_J outputs followed by the mat_K outputs, src[t] = 0
+ * for a Coulomb and 1 for an exchange contribution; with cJ = 2, cK = -1 (D
+ * without the spin factor 2) this is the closed-shell G = 2J - K.  Replaces the
+ * Fock combination of VeloxChem's host driver (not in the reference tree). */
+int vlxmi_fock_assemble(const double* values, const uint32_t* perm, const uint8_t* src, const uint32_t* seg_off,
+                        const uint32_t* seg_out, uint32_t nseg, double cJ, double cK, double* out, void* stream);
+
 /* p[0..n) = 0 by a kernel (safe inside HIP-graph capture). */
 int vlxmi_fill_zero(double* p, uint64_t n, void* stream);
 

@@ -56,17 +56,41 @@ def test_sources_use_table_values():
     srcs = B.sources()
     checked = 0
     for fn, kt in tiles.items():
-        kind = "J" if fn.startswith("j") else "K"
+        pre = fn.split("_")[0]
+        fam = ("J" if pre.startswith("j") else "K") + ("b" if "b" in pre[1:] else "")
         cls = fn.split("_")[1].split(".")[0]
         m = re.search(r"constexpr int KT = (\d+)", srcs[fn])
-        want = emit.LDS_BOYS_KT if (cls in blds[kind] and not fn.startswith("schwarz")) else kt
+        want = emit.LDS_BOYS_KT if (cls in blds[fam] and not fn.startswith("schwarz")) else kt
         assert m and int(m.group(1)) == want, fn
         checked += 1
     assert checked == len(tiles)
-    for kind, pre in (("J", "j"), ("K", "k")):
-        for cls in blds[kind]:
-            s = srcs["%s_%s.hip" % (pre, cls)]
-            assert "vlxmi::boys_lds<" in s and "vlxmi::boys<" not in s and "__syncthreads();" in s
+    for fam, pres in (("J", ("j", "jm")), ("K", ("k", "km")), ("Jb", ("jb", "jbm")), ("Kb", ("kb", "kbm"))):
+        for cls in blds[fam]:
+            for pre in pres:
+                s = srcs["%s_%s.hip" % (pre, cls)]
+                assert "vlxmi::boys_lds<" in s and "vlxmi::boys<" not in s and "__syncthreads();" in s
+
+
+def test_waves_per_workgroup_table():
+    """Classes run with several wavefronts per workgroup only where the table
+    says so; those kernels synchronise each wavefront on its own after the one
+    workgroup barrier that publishes the LDS Boys rows, and are launched with
+    64 * waves threads."""
+    wp = B.load_wpg()
+    srcs = B.sources()
+    for fam, pres in (("J", ("j", "jm")), ("K", ("k", "km")), ("Jb", ("jb", "jbm")), ("Kb", ("kb", "kbm"))):
+        for pre in pres:
+            for fn, s in srcs.items():
+                if not fn.startswith(pre + "_"):
+                    continue
+                cls = fn.split("_")[1].split(".")[0]
+                w = int(wp[fam].get(cls, 1))
+                if w == 1:
+                    assert "vlxmi::wave_sync()" not in s and "WPG" not in s, fn
+                else:
+                    assert "constexpr int WPG = %d;" % w in s and "vlxmi::wave_sync();" in s, fn
+                    assert s.count("__syncthreads();") <= 1, fn
+                    assert "dim3(%d)" % (64 * w) in s, fn
 
 
 def test_mixed_kernels_have_one_body_per_loop():

@@ -202,6 +202,35 @@ extern "C" int vlxmi_segment_sum(const double* values, const uint32_t* perm, con
     return (int)hipGetLastError();
 }
 
+// Fock-matrix assembly: out[seg_out[s]] = sum over t in [seg_off[s], seg_off[s+1])
+// of coef[src[t]] * values[perm[t]], in list order (one writer per output, no
+// atomics).  With the Coulomb and exchange per-pair outputs concatenated in
+// ``values`` and src = 0 / 1 selecting c_J / c_K, one pass gives the
+// two-electron part c_J J + c_K K of the Fock matrix in the Cartesian AO basis.
+__global__ void fock_assemble_kernel(const double* __restrict__ values, const uint32_t* __restrict__ perm,
+                                     const uint8_t* __restrict__ src, const uint32_t* __restrict__ seg_off,
+                                     const uint32_t* __restrict__ seg_out, uint32_t nseg, double cJ, double cK,
+                                     double* __restrict__ out)
+{
+    const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= nseg) return;
+    double acc = 0.0;
+    const uint32_t e = seg_off[s + 1];
+    for (uint32_t t = seg_off[s]; t < e; ++t) acc += (src[t] ? cK : cJ) * values[perm[t]];
+    out[seg_out[s]] = acc;
+}
+
+extern "C" int vlxmi_fock_assemble(const double* values, const uint32_t* perm, const uint8_t* src,
+                                   const uint32_t* seg_off, const uint32_t* seg_out, uint32_t nseg, double cJ,
+                                   double cK, double* out, void* stream)
+{
+    if (nseg == 0) return VLXMI_OK;
+    if (!values || !perm || !src || !seg_off || !seg_out || !out) return VLXMI_ERR_ARG;
+    hipLaunchKernelGGL(fock_assemble_kernel, dim3((nseg + 255) / 256), dim3(256), 0, (hipStream_t)stream, values,
+                       perm, src, seg_off, seg_out, nseg, cJ, cK, out);
+    return (int)hipGetLastError();
+}
+
 // p[0..n) = 0 with a kernel (graph-capture safe: records no memset node).
 __global__ void fill_zero_kernel(double* __restrict__ p, uint64_t n)
 {

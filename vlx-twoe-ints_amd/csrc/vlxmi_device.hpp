@@ -75,6 +75,16 @@ extern "C" int vlxmi_ket_split_reduce(const double* part, uint32_t nsplit, uint3
 
 namespace vlxmi {
 
+// Orders one wavefront's LDS accesses around this point: the ket tile a
+// wavefront stages is read only by its own lanes, so kernels that run several
+// independent wavefronts per workgroup synchronise each wavefront on its own.
+__device__ __forceinline__ void wave_sync()
+{
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
 // 1/x and 1/sqrt(x) for positive, normal x well inside the double range (the
 // Gaussian exponent sums and Boys arguments these are used on): the hardware
 // estimate refined by two Newton steps, each of which squares the relative

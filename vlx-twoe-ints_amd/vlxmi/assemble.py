@@ -63,3 +63,34 @@ class Assembler:
 def zero_(dev, t):
     check(lib().vlxmi_fill_zero(t.data_ptr(), t.numel(), dev.stream_ptr()), "fill_zero")
     return t
+
+
+class FockAssembler:
+    """The two-electron part of the Fock matrix, G = c_J J + c_K K, in one
+    deterministic pass (``vlxmi_fock_assemble``) from the per-pair Coulomb and
+    exchange outputs: the two value -> output maps are merged into one plan
+    sorted by output (Coulomb contributions first, then exchange, each in
+    value order), each contribution tagged with its coefficient."""
+
+    def __init__(self, dev, j_value_idx, j_out_idx, nj_values, k_value_idx, k_out_idx, nout):
+        vi = np.concatenate([np.asarray(j_value_idx, np.int64), np.asarray(k_value_idx, np.int64) + nj_values])
+        oi = np.concatenate([np.asarray(j_out_idx, np.int64), np.asarray(k_out_idx, np.int64)])
+        src = np.concatenate([np.zeros(len(j_value_idx), np.uint8), np.ones(len(k_value_idx), np.uint8)])
+        assert vi.size < 2 ** 32 and nout < 2 ** 32
+        order = np.lexsort((vi, src, oi))
+        vo, oo, so = vi[order], oi[order], src[order]
+        starts = np.nonzero(np.r_[True, oo[1:] != oo[:-1]])[0] if oo.size else np.zeros(0, dtype=np.int64)
+        self.dev = dev
+        self.nseg = int(starts.size)
+        self.nout = int(nout)
+        self.perm = dev.tensor(vo.astype(np.uint32))
+        self.src = dev.tensor(so)
+        self.seg_off = dev.tensor(np.r_[starts, oo.size].astype(np.uint32))
+        self.seg_out = dev.tensor(oo[starts].astype(np.uint32))
+
+    def run(self, values, out, cJ, cK):
+        """out[...] = c_J J + c_K K (elements without contributions untouched)."""
+        check(lib().vlxmi_fock_assemble(values.data_ptr(), self.perm.data_ptr(), self.src.data_ptr(),
+                                        self.seg_off.data_ptr(), self.seg_out.data_ptr(), self.nseg, float(cJ),
+                                        float(cK), out.data_ptr(), self.dev.stream_ptr()), "fock_assemble")
+        return out

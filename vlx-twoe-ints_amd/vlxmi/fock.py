@@ -119,6 +119,7 @@ class FockEngine:
             oi += [f[sel1] * n + s[sel1], s[sel2] * n + f[sel2]]
             off += jl.count
         self.jasm = Assembler(d, np.concatenate(vi), np.concatenate(oi), n * n)
+        self._jmap = (np.concatenate(vi), np.concatenate(oi), off)
         vi, oi, off = [], [], 0
         self.k_order = [pc for pc in P.PAIR_CLASSES if pc in fl.iklists]
         for pc in self.k_order:
@@ -135,6 +136,8 @@ class FockEngine:
             oi += [ai * n + ak, ak[sel2] * n + ai[sel2]]
             off += int(ik.i.size)
         self.kasm = Assembler(d, np.concatenate(vi), np.concatenate(oi), n * n)
+        self._kmap = (np.concatenate(vi), np.concatenate(oi))
+        self.fasm = None           # FockAssembler, made on the first fock() call
 
     def _cat(self, m, order, counts):
         t = self.torch
@@ -339,6 +342,34 @@ class FockEngine:
                                            Cc["blk_max"].data_ptr(), Cc["blk_displs"].data_ptr(), Cc["blk_counts"].data_ptr(),
                                            self.thr / maxD, out.data_ptr(), self.dev.stream_ptr()), "count_k_quartets")
         return int(out.item())
+
+    def fock(self, D_sph, H=None, cJ=2.0, cK=-1.0):
+        """F = H + c_J J + c_K K in the output basis for density D (device
+        tensors; H optional, the core Hamiltonian is not computed here).  The
+        Coulomb and exchange per-pair outputs are assembled into c_J J + c_K K
+        by one deterministic pass (FockAssembler), then transformed to the
+        output basis; the defaults give the closed-shell 2J - K for D without
+        the spin factor 2."""
+        from .assemble import FockAssembler
+        t = self.torch
+        n = self.naos
+        if self.fasm is None:
+            jv, jo, nj = self._jmap
+            kv, ko = self._kmap
+            self.fasm = FockAssembler(self.dev, jv, jo, nj, kv, ko, n * n)
+        if self.precision == "mixed":
+            self.reset_counts()
+        Dc = self.cart_density(D_sph)
+        mJ = self.run_J(Dc)
+        mK = self.run_K(Dc)
+        vals = t.cat([self._cat(mJ, self.j_order, {pc: self.jl[pc]["count"] for pc in self.j_order}),
+                      self._cat(mK, self.k_order, {pc: self.ik[pc]["count"] for pc in self.k_order})])
+        G = t.zeros(n * n, dtype=t.float64, device=self.dev.device)
+        self.fasm.run(vals, G, cJ, cK)
+        F = self.T @ G.view(n, n) @ self.T.t()
+        if H is not None:
+            F = F + (H if t.is_tensor(H) else self.dev.tensor(np.asarray(H, dtype=np.float64)))
+        return F
 
     def reset_counts(self):
         """Zero the FP64/FP32 quartet counters of the _mixed kernels."""

@@ -61,13 +61,31 @@ DORDER_TABLE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "k_dorde
 BOYS_LDS_TABLE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "boys_lds.json")
 
 
+# light classes whose kernels run several wavefronts per workgroup (sharing the
+# LDS Boys rows), chosen from per-class timings
+WPG_TABLE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "wpg.json")
+
+
+FAMILIES = ("J", "K", "Jb", "Kb")     # class launchers (+ _mixed), batched (+ _mixed)
+
+
+def load_wpg():
+    """{family: {class: waves per workgroup}} (families J, K, Jb, Kb); classes
+    absent run one wavefront per workgroup."""
+    tab = {}
+    if os.path.exists(WPG_TABLE):
+        with open(WPG_TABLE) as f:
+            tab = json.load(f)
+    return {fam: dict(tab.get(fam, {})) for fam in FAMILIES}
+
+
 def load_boys_lds():
-    """{"J": set of classes, "K": set of classes} using the LDS Boys rows."""
+    """{family: set of classes} using the LDS Boys rows (families J, K, Jb, Kb)."""
+    tab = {}
     if os.path.exists(BOYS_LDS_TABLE):
         with open(BOYS_LDS_TABLE) as f:
             tab = json.load(f)
-        return {"J": set(tab.get("J", [])), "K": set(tab.get("K", []))}
-    return {"J": set(), "K": set()}
+    return {fam: set(tab.get(fam, [])) for fam in FAMILIES}
 LLVM = "/opt/rocm/llvm/bin"
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = "gfx950"
@@ -102,6 +120,7 @@ def sources(tiles=None, layouts=None):
     jlayouts = load_layouts(J_LAYOUT_TABLE)
     dorders = load_layouts(DORDER_TABLE)
     blds = load_boys_lds()
+    wpg = load_wpg()
     out = {}
     inc = '#include "vlxmi_device.hpp"\n\n'
     for ab, cd in j_classes():
@@ -109,14 +128,17 @@ def sources(tiles=None, layouts=None):
         for pre, kw in (("j", {}), ("jm", dict(mixed=True)), ("jb", dict(batched=True)),
                         ("jbm", dict(batched=True, mixed=True))):
             fn = "%s_%s.hip" % (pre, cls)
+            fam = "Jb" if kw.get("batched") else "J"
             out[fn] = inc + emit.emit_j(cls, kt=tiles.get(fn), layout=jlayouts.get(cls, "reg"),
-                                        lds_boys=cls in blds["J"], **kw)
+                                        lds_boys=cls in blds[fam], wpg=int(wpg[fam].get(cls, 1)), **kw)
     for cls in k_classes():
         for pre, kw in (("k", {}), ("km", dict(mixed=True)), ("kb", dict(batched=True)),
                         ("kbm", dict(batched=True, mixed=True))):
             fn = "%s_%s.hip" % (pre, cls)
+            fam = "Kb" if kw.get("batched") else "K"
             out[fn] = inc + emit.emit_k(cls, kt=tiles.get(fn), layout=layouts.get(cls, "reg"),
-                                        dorder=dorders.get(cls, "jl"), lds_boys=cls in blds["K"], **kw)
+                                        dorder=dorders.get(cls, "jl"), lds_boys=cls in blds[fam],
+                                        wpg=int(wpg[fam].get(cls, 1)), **kw)
     for pc in PAIR_CLASSES:
         out["schwarz_%s.hip" % pc] = inc + emit.emit_schwarz(pc)
     out["jdispatch.hip"] = inc + jdispatch_text()

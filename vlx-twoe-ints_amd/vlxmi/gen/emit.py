@@ -279,6 +279,20 @@ def ket_loops(ind, act, screen, bound, header, body, mixed, lo, count=None):
     return o
 
 
+def wave_decls(wpg, tile="TILE_SIZE_J"):
+    """Lane and wavefront indices; with several wavefronts per workgroup (wpg)
+    each wavefront works on its own items and keeps its own LDS ket tile."""
+    if wpg == 1:
+        return ["    const uint32_t lane = threadIdx.x;"]
+    return ["    const uint32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6;"]
+
+
+def sync(wpg):
+    """The barrier between staging the ket tile in LDS and reading it: the
+    workgroup's one wavefront, or (several wavefronts) this wavefront only."""
+    return "__syncthreads();" if wpg == 1 else "vlxmi::wave_sync();"
+
+
 def count_decl():
     """Per-lane quartet counters of a mixed kernel (FP64 and FP32 evaluations)."""
     return ["    uint32_t nq64 = 0, nq32 = 0;"]
@@ -290,7 +304,7 @@ def count_flush(ind="    "):
     return [ind + "if (quartet_counts != nullptr) {",
             ind + "    #pragma unroll",
             ind + "    for (int off = 32; off > 0; off >>= 1) { nq64 += __shfl_xor(nq64, off, 64); nq32 += __shfl_xor(nq32, off, 64); }",
-            ind + "    if (threadIdx.x == 0) {",
+            ind + "    if ((threadIdx.x & 63) == 0) {",
             ind + "        atomicAdd(&quartet_counts[0], (unsigned long long)nq64);",
             ind + "        atomicAdd(&quartet_counts[1], (unsigned long long)nq32);",
             ind + "    }",
@@ -389,7 +403,7 @@ def ket_record_bytes(kind, cls):
     return rec * 8 + (NC[d] * 4 if kind == "K" else 0)
 
 
-def emit_j(cls, mixed=False, batched=False, kt=None, layout="reg", lds_boys=False):
+def emit_j(cls, mixed=False, batched=False, kt=None, layout="reg", lds_boys=False, wpg=1):
     """Source of computeCoulombFock{CLS} (or its _mixed / _batched variants) and its
     C-ABI launcher.  The batched variant is persistent: workgroup w processes
     work items w, w + gridDim.x, ...; item t = (bra block begin, bra block end
@@ -414,8 +428,9 @@ def emit_j(cls, mixed=False, batched=False, kt=None, layout="reg", lds_boys=Fals
     # partial sums at ket_partial[y * count_local + e] for the launcher's
     # deterministic reduction.
     KP = ["double* __restrict__ ket_partial"]
+    LB = "TILE_SIZE_J" if wpg == 1 else str(64 * wpg)
     if batched:
-        o.append("__global__ void __launch_bounds__(TILE_SIZE_J)")
+        o.append("__global__ void __launch_bounds__(%s)" % LB)
         o.append("%s(%s)" % (name, ",\n    ".join(sig)))
     else:
         o.append("static __device__ __forceinline__ void")
@@ -423,22 +438,31 @@ def emit_j(cls, mixed=False, batched=False, kt=None, layout="reg", lds_boys=Fals
     o.append("{")
     o.append("    constexpr int NA = %d, NB = %d, NC = %d, ND = %d, NAB = %d, NCD = %d;" % (na, nb, nc, nd, nab, ncd))
     o.append("    constexpr int KT = %d, REC = %d;" % (LDS_BOYS_KT if lds_boys else (kt or KET_TILE), rec))
-    o.append("    __shared__ double ket[KT * REC];")
-    o.append("    const uint32_t lane = threadIdx.x;")
+    if wpg == 1:
+        o.append("    __shared__ double ket[KT * REC];")
+    else:
+        o.append("    constexpr int WPG = %d;" % wpg)
+        o.append("    __shared__ double ket_all[WPG * KT * REC];")
+    o += wave_decls(wpg)
+    if wpg > 1:
+        o.append("    double* ket = ket_all + wave * (KT * REC);")
     if lds_boys:
         o += lds_boys_setup(L)
     o.append("    const uint32_t nbra = %s_prim_pair_count_local / NAB;" % ab)
     o.append("    const uint32_t nket = %s_prim_pair_count / NCD;" % cd)
     if batched:
         o += count_decl()
-        o.append("    for (uint32_t w = blockIdx.x; w < nwork; w += gridDim.x) {")
+        if wpg == 1:
+            o.append("    for (uint32_t w = blockIdx.x; w < nwork; w += gridDim.x) {")
+        else:
+            o.append("    for (uint32_t w = blockIdx.x * WPG + wave; w < nwork; w += gridDim.x * WPG) {")
         o.append("    const uint32_t bra_end = work[4 * w + 1] < nbra ? work[4 * w + 1] : nbra;")
         o.append("    const uint32_t bb = work[4 * w] + lane;")
         o.append("    const uint32_t k_begin = work[4 * w + 2];")
         o.append("    const uint32_t k_end = work[4 * w + 3] < nket ? work[4 * w + 3] : nket;")
         o.append("    const bool own = bb < bra_end;")
     else:
-        o.append("    const uint32_t bb = blockIdx.x * TILE_SIZE_J + lane;")
+        o.append("    const uint32_t bb = %s * TILE_SIZE_J + lane;" % ("blockIdx.x" if wpg == 1 else "(blockIdx.x * WPG + wave)"))
         o.append("    const uint32_t chunk = (nket + gridDim.y - 1) / gridDim.y;")
         o.append("    const uint32_t k_begin = blockIdx.y * chunk;")
         o.append("    const uint32_t k_end = (k_begin + chunk < nket) ? (k_begin + chunk) : nket;")
@@ -455,6 +479,7 @@ def emit_j(cls, mixed=False, batched=False, kt=None, layout="reg", lds_boys=Fals
     # digestion adding each term with one LDS f64 add and reading the
     # threshold per use, which frees 4*NAB registers for the recursion.
     jlds = len(jsl) > 1 and layout == "lds"
+    assert not (jlds and wpg > 1), "the LDS accumulator layout keeps one wavefront per workgroup"
     if jlds:
         o.append("    __shared__ double ltau[NAB * TILE_SIZE_J], lacc[NAB * TILE_SIZE_J];")
         o.append("    double* tau = ltau;")
@@ -490,7 +515,7 @@ def emit_j(cls, mixed=False, batched=False, kt=None, layout="reg", lds_boys=Fals
             o += count_decl()
     o.append("    for (uint32_t t0 = k_begin; t0 < k_end; t0 += KT) {")
     o.append("        if (!__any(active)) break;")
-    o.append("        __syncthreads();")
+    o.append("        " + sync(wpg))
     o.append("        {")
     o.append("            const uint32_t kb = t0 + lane;")
     o.append("            double* R = ket + lane * REC;")
@@ -523,7 +548,7 @@ def emit_j(cls, mixed=False, batched=False, kt=None, layout="reg", lds_boys=Fals
     o.append("                R[0] = beta;")
     o.append("            }")
     o.append("        }")
-    o.append("        __syncthreads();")
+    o.append("        " + sync(wpg))
     o.append("        const uint32_t nt = (k_end - t0 < (uint32_t)KT) ? (k_end - t0) : (uint32_t)KT;")
 
     def jbody(tyq, ind, groups):
@@ -584,13 +609,13 @@ def emit_j(cls, mixed=False, batched=False, kt=None, layout="reg", lds_boys=Fals
     args = _argnames(sig)
     if not batched:
         if not mixed:
-            o.append("__global__ void __launch_bounds__(TILE_SIZE_J)")
+            o.append("__global__ void __launch_bounds__(%s)" % LB)
             o.append("%s(%s)" % (name, ",\n    ".join(sig)))
             o.append("{")
             o.append("    %s_body(%s, nullptr);" % (name, ", ".join(args)))
             o.append("}")
             o.append("")
-        o.append("__global__ void __launch_bounds__(TILE_SIZE_J)")
+        o.append("__global__ void __launch_bounds__(%s)" % LB)
         o.append("%s_ks(%s)" % (name, ",\n    ".join(sig + KP)))
         o.append("{")
         o.append("    %s_body(%s, ket_partial);" % (name, ", ".join(args)))
@@ -608,9 +633,10 @@ def emit_j(cls, mixed=False, batched=False, kt=None, layout="reg", lds_boys=Fals
         o += checks
         o.append("    if (nwork == 0) return VLXMI_OK;")
         o.append("    if (!work) return VLXMI_ERR_ARG;")
-        o.append("    if (n_workgroups == 0 || n_workgroups > nwork) n_workgroups = nwork;")
+        o.append("    const uint32_t max_wg = (nwork + %d) / %d;" % (wpg - 1, wpg))
+        o.append("    if (n_workgroups == 0 || n_workgroups > max_wg) n_workgroups = max_wg;")
         o.append("    dim3 grid(n_workgroups);")
-        o.append("    hipLaunchKernelGGL(%s, grid, dim3(TILE_SIZE_J), 0, (hipStream_t)stream, %s);" % (name, ", ".join(args)))
+        o.append("    hipLaunchKernelGGL(%s, grid, dim3(%d), 0, (hipStream_t)stream, %s);" % (name, 64 * wpg, ", ".join(args)))
         o.append("    return (int)hipGetLastError();")
         o.append("}")
         return "\n".join(o) + "\n"
@@ -624,18 +650,18 @@ def emit_j(cls, mixed=False, batched=False, kt=None, layout="reg", lds_boys=Fals
              % (name, ",\n    ".join(lsig)))
     o.append("{")
     o += checks + clamp
-    o.append("    dim3 grid((nbra + TILE_SIZE_J - 1) / TILE_SIZE_J, ket_split);")
+    o.append("    dim3 grid((nbra + %d) / %d, ket_split);" % (64 * wpg - 1, 64 * wpg))
     o.append("    if (ket_split > 1) {")
     o.append("        if (!part) return VLXMI_ERR_ARG;")
-    o.append("        hipLaunchKernelGGL(%s_ks, grid, dim3(TILE_SIZE_J), 0, (hipStream_t)stream, %s, part);" % (name, ", ".join(args)))
+    o.append("        hipLaunchKernelGGL(%s_ks, grid, dim3(%d), 0, (hipStream_t)stream, %s, part);" % (name, 64 * wpg, ", ".join(args)))
     o.append("        const int rc = (int)hipGetLastError();")
     o.append("        if (rc != 0) return rc;")
     o.append("        return vlxmi_ket_split_reduce(part, ket_split, %s_prim_pair_count_local, mat_J, stream);" % ab)
     o.append("    }")
     if mixed:
-        o.append("    hipLaunchKernelGGL(%s_ks, grid, dim3(TILE_SIZE_J), 0, (hipStream_t)stream, %s, nullptr);" % (name, ", ".join(args)))
+        o.append("    hipLaunchKernelGGL(%s_ks, grid, dim3(%d), 0, (hipStream_t)stream, %s, nullptr);" % (name, 64 * wpg, ", ".join(args)))
     else:
-        o.append("    hipLaunchKernelGGL(%s, grid, dim3(TILE_SIZE_J), 0, (hipStream_t)stream, %s);" % (name, ", ".join(args)))
+        o.append("    hipLaunchKernelGGL(%s, grid, dim3(%d), 0, (hipStream_t)stream, %s);" % (name, 64 * wpg, ", ".join(args)))
     o.append("    return (int)hipGetLastError();")
     o.append("}")
     o.append("")
@@ -666,7 +692,7 @@ def emit_j(cls, mixed=False, batched=False, kt=None, layout="reg", lds_boys=Fals
 # K kernel
 # --------------------------------------------------------------------------
 
-def emit_k(cls, mixed=False, batched=False, kt=None, layout="reg", dorder="jl", lds_boys=False):
+def emit_k(cls, mixed=False, batched=False, kt=None, layout="reg", dorder="jl", lds_boys=False, wpg=1):
     """Source of computeExchangeFock{CLS} (or its _mixed / _batched variants) and
     its launcher.  The batched variant is persistent over a list of (i,k) blocks
     of lists that concatenate many molecules; per block, blk_mol gives the
@@ -697,13 +723,19 @@ def emit_k(cls, mixed=False, batched=False, kt=None, layout="reg", dorder="jl", 
     sig = k_signature(cls) + (bsig if batched else []) + tail_args(mixed, batched)
     ty = "double"
     o = []
-    o.append("__global__ void __launch_bounds__(TILE_SIZE_K)")
+    assert not (heavy and wpg > 1), "the heavy layouts keep one wavefront per workgroup"
+    o.append("__global__ void __launch_bounds__(%s)" % ("TILE_SIZE_K" if wpg == 1 else str(64 * wpg)))
     o.append("%s(%s)" % (name, ",\n    ".join(sig)))
     o.append("{")
     o.append("    constexpr int NA = %d, NB = %d, NC = %d, ND = %d, NAC = %d, NAB = %d, NCD = %d;" % (na, nb, nc, nd, nac, nab, ncd))
     o.append("    constexpr int KT = %d, REC = %d;" % (LDS_BOYS_KT if lds_boys else (kt or (32 if heavy else KET_TILE)), rec))
-    o.append("    __shared__ double ket[KT * REC];")
-    o.append("    __shared__ uint32_t ket_cg[KT * ND];")
+    if wpg == 1:
+        o.append("    __shared__ double ket[KT * REC];")
+        o.append("    __shared__ uint32_t ket_cg[KT * ND];")
+    else:
+        o.append("    constexpr int WPG = %d;" % wpg)
+        o.append("    __shared__ double ket_all[WPG * KT * REC];")
+        o.append("    __shared__ uint32_t ket_cg_all[WPG * KT * ND];")
     # heavy classes keep K accumulators in LDS when they fit beside the
     # thresholds, S_ab and an 8-record ket tile within one wave's LDS share
     # Heavy classes have two layouts, chosen per class by measurement:
@@ -722,14 +754,20 @@ def emit_k(cls, mixed=False, batched=False, kt=None, layout="reg", dorder="jl", 
         # per-lane K accumulators in LDS ([component][lane]); a pass keeps its
         # own NA accumulators in registers for the duration of its ket loop
         o.append("    __shared__ double lacc[NAC * TILE_SIZE_K];")
-    o.append("    const uint32_t lane = threadIdx.x;")
+    o += wave_decls(wpg)
+    if wpg > 1:
+        o.append("    double* ket = ket_all + wave * (KT * REC);")
+        o.append("    uint32_t* ket_cg = ket_cg_all + wave * (KT * ND);")
     if lds_boys:
         o += lds_boys_setup(L)
     if mixed or batched:
         o += count_decl()
     if batched:
         o.append("    (void)%s_max_D;" % bd)
-        o.append("    for (uint32_t w = blockIdx.x; w < nlist; w += gridDim.x) {")
+        if wpg == 1:
+            o.append("    for (uint32_t w = blockIdx.x; w < nlist; w += gridDim.x) {")
+        else:
+            o.append("    for (uint32_t w = blockIdx.x * WPG + wave; w < nlist; w += gridDim.x * WPG) {")
         o.append("    const uint32_t blk = blk_list[w];")
         o.append("    if ((blk + 1) * NAC > pair_inds_count_for_K_%s) continue;" % ac)
         o.append("    const uint32_t mol = blk_mol[blk];")
@@ -738,7 +776,7 @@ def emit_k(cls, mixed=False, batched=False, kt=None, layout="reg", dorder="jl", 
         o.append("    const double* __restrict__ Dm = mat_D_full_AO + mol_D_off[mol];")
         o.append("    const uint32_t nao = mol_naos[mol], aoff = mol_ao_off[mol];")
     else:
-        o.append("    const uint32_t blk = blockIdx.x;")
+        o.append("    const uint32_t blk = %s;" % ("blockIdx.x" if wpg == 1 else "blockIdx.x * WPG + wave"))
         o.append("    if ((blk + 1) * NAC > pair_inds_count_for_K_%s) return;" % ac)
         o.append("    const double maxD = %s_max_D;" % bd)
         o.append("    if (!(maxD > 0.0)) return;")
@@ -822,7 +860,7 @@ def emit_k(cls, mixed=False, batched=False, kt=None, layout="reg", dorder="jl", 
         o.append("        bool lo_mode = false;")
     o.append("        for (uint32_t t0 = 0; t0 < nL; t0 += KT) {")
     o.append("            if (!__any(active)) break;")
-    o.append("            __syncthreads();")
+    o.append("            " + sync(wpg))
     o.append("            {")
     o.append("                const uint32_t lp = t0 + lane;")
     o.append("                double* R = ket + lane * REC;")
@@ -859,7 +897,7 @@ def emit_k(cls, mixed=False, batched=False, kt=None, layout="reg", dorder="jl", 
     o.append("                    }")
     o.append("                }")
     o.append("            }")
-    o.append("            __syncthreads();")
+    o.append("            " + sync(wpg))
     o.append("            const uint32_t nt = (nL - t0 < (uint32_t)KT) ? (nL - t0) : (uint32_t)KT;")
     assert dorder in ("jl", "lj")
 
@@ -1016,7 +1054,7 @@ def emit_k(cls, mixed=False, batched=False, kt=None, layout="reg", dorder="jl", 
     o.append("        if (lane == 0) mat_K[e0 + t] += v;")
     o.append("    }")
     if batched:
-        o.append("    __syncthreads();")
+        o.append("    " + sync(wpg))
         o.append("    }")
     if mixed or batched:
         o += count_flush()
@@ -1048,10 +1086,11 @@ def emit_k(cls, mixed=False, batched=False, kt=None, layout="reg", dorder="jl", 
     if batched:
         o.append("    if (nlist == 0) return VLXMI_OK;")
         o.append("    if (!blk_list || !blk_mol || !mol_maxD || !mol_D_off || !mol_naos || !mol_ao_off) return VLXMI_ERR_ARG;")
-        o.append("    if (n_workgroups == 0 || n_workgroups > nlist) n_workgroups = nlist;")
-        o.append("    hipLaunchKernelGGL(%s, dim3(n_workgroups), dim3(TILE_SIZE_K), 0, (hipStream_t)stream, %s);" % (name, ", ".join(args)))
+        o.append("    const uint32_t max_wg = (nlist + %d) / %d;" % (wpg - 1, wpg))
+        o.append("    if (n_workgroups == 0 || n_workgroups > max_wg) n_workgroups = max_wg;")
+        o.append("    hipLaunchKernelGGL(%s, dim3(n_workgroups), dim3(%d), 0, (hipStream_t)stream, %s);" % (name, 64 * wpg, ", ".join(args)))
     else:
-        o.append("    hipLaunchKernelGGL(%s, dim3(nblk), dim3(TILE_SIZE_K), 0, (hipStream_t)stream, %s);" % (name, ", ".join(args)))
+        o.append("    hipLaunchKernelGGL(%s, dim3((nblk + %d) / %d), dim3(%d), 0, (hipStream_t)stream, %s);" % (name, wpg - 1, wpg, 64 * wpg, ", ".join(args)))
     o.append("    return (int)hipGetLastError();")
     o.append("}")
     return "\n".join(o) + "\n"

@@ -140,6 +140,42 @@ def counters_markdown(pmc_dir, per_class_path, top=25, copy_bytes=512 << 20):
     return "\n".join(o) + "\n"
 
 
+def busy_markdown(path, top=30):
+    """Per-kernel issue and memory-pipeline figures of one counter pass
+    (SQ_WAVES, SQ_WAVE_CYCLES, SQ_INSTS_VALU, SQ_ACTIVE_INST_VALU,
+    SQ_INSTS_VMEM_RD, SQ_INSTS_LDS, SQ_WAIT_INST_ANY, TA_TA_BUSY, TD_TD_BUSY,
+    GRBM_GUI_ACTIVE), averaged over each kernel's dispatches:
+
+    * VALU busy: SQ_ACTIVE_INST_VALU / SQ_WAVE_CYCLES, the share of a wave's
+      life spent issuing vector ALU instructions;
+    * waiting: SQ_WAIT_INST_ANY / SQ_WAVE_CYCLES, the share spent waiting for
+      any instruction's inputs (memory, mostly);
+    * TA / TD busy: summed over the per-CU units, divided by
+      (GRBM_GUI_ACTIVE / 8 XCDs) x 256 CUs."""
+    d = counters_from(path)
+    agg = {}
+    for (_, name), v in d.items():
+        a = agg.setdefault(name, {"_n": 0})
+        a["_n"] += 1
+        for k, x in v.items():
+            a[k] = a.get(k, 0.0) + x
+    rows = sorted(agg.items(), key=lambda kv: -kv[1].get("_t", 0.0))[:top]
+    o = ["| kernel | dispatches | ms / dispatch | waves | VALU instr / wave | VMEM rd / wave | LDS instr / wave "
+         "| VALU busy | waiting | TA busy | TD busy |",
+         "|---|---:|---:|---:|---:|---:|---:|---:|---:|---:|---:|"]
+    for name, a in rows:
+        n = a["_n"]
+        w = a.get("SQ_WAVES", 0.0) or 1.0
+        cyc = a.get("SQ_WAVE_CYCLES", 0.0) or 1.0
+        g = a.get("GRBM_GUI_ACTIVE", 0.0) / 8.0 * 256.0
+        o.append("| %s | %d | %.3f | %.0f | %.1f | %.2f | %.2f | %.2f | %.2f | %.2f | %.2f |" % (
+            name, n, 1e3 * a.get("_t", 0.0) / n, w / n, a.get("SQ_INSTS_VALU", 0.0) / w,
+            a.get("SQ_INSTS_VMEM_RD", 0.0) / w, a.get("SQ_INSTS_LDS", 0.0) / w,
+            a.get("SQ_ACTIVE_INST_VALU", 0.0) / cyc, a.get("SQ_WAIT_INST_ANY", 0.0) / cyc,
+            a.get("TA_TA_BUSY", 0.0) / g if g else 0.0, a.get("TD_TD_BUSY", 0.0) / g if g else 0.0))
+    return "\n".join(o) + "\n"
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("result", nargs="?")
@@ -147,8 +183,11 @@ def main():
     ap.add_argument("--top", type=int, default=30)
     ap.add_argument("--counters", default=None, help="directory holding the mem/fetch/write counter passes")
     ap.add_argument("--per-class", default=None, help="per-class JSON written by bench.py --per-class-out")
+    ap.add_argument("--busy", default=None, help="run_counter_collection.csv of one issue/memory counter pass")
     a = ap.parse_args()
-    if a.counters:
+    if a.busy:
+        print(busy_markdown(a.busy, a.top), end="")
+    elif a.counters:
         print(counters_markdown(a.counters, a.per_class, a.top), end="")
     else:
         print(markdown(a.result, a.command, a.top), end="")

@@ -95,6 +95,7 @@ def lib():
     L.vlxmi_version.restype = C.c_int
     L.vlxmi_segment_sum.argtypes = [C.c_void_p] * 4 + [C.c_uint32, C.c_void_p, C.c_void_p]
     L.vlxmi_fill_zero.argtypes = [C.c_void_p, C.c_uint64, C.c_void_p]
+    L.vlxmi_fock_assemble.argtypes = [C.c_void_p] * 5 + [C.c_uint32, C.c_double, C.c_double, C.c_void_p, C.c_void_p]
     L.vlxmi_ket_workspace.argtypes = [C.c_size_t, C.c_void_p, C.POINTER(C.c_void_p)]
     L.vlxmi_release_workspaces.argtypes = []
     L.vlxmi_ket_split_reduce.argtypes = [C.c_void_p, C.c_uint32, C.c_uint32, C.c_void_p, C.c_void_p]
@@ -135,7 +136,7 @@ def exported_symbols():
              "vlxmi_coulomb_fock_noalloc", "vlxmi_noalloc_scratch_doubles", "vlxmi_j_list_to_blocks",
              "vlxmi_ik_list_to_blocks", "vlxmi_k_csr_to_blocks", "vlxmi_scatter_add", "vlxmi_validate_j_list",
              "vlxmi_validate_ik_list", "vlxmi_validate_k_csr", "vlxmi_set_list_validation", "vlxmi_list_validation",
-             "vlxmi_pair_candidates", "vlxmi_block_absmax", "vlxmi_kcsr_fill"]
+             "vlxmi_pair_candidates", "vlxmi_block_absmax", "vlxmi_kcsr_fill", "vlxmi_fock_assemble"]
     names += ["vlxmi_launch_computeCoulombFock" + (ab + cd).upper() for ab, cd in j_classes()]
     names += ["vlxmi_launch_computeExchangeFock" + c.upper() for c in k_classes()]
     names += ["vlxmi_launch_computeCoulombFock" + (ab + cd).upper() + "_mixed" for ab, cd in j_classes()]
@@ -181,7 +182,8 @@ class Device:
         a = np.ascontiguousarray(a)
         if dtype is None:
             dtype = {np.dtype(np.float64): t.float64, np.dtype(np.uint32): t.int32,
-                     np.dtype(np.int64): t.int64, np.dtype(np.int32): t.int32, np.dtype(bool): t.bool}[a.dtype]
+                     np.dtype(np.int64): t.int64, np.dtype(np.int32): t.int32, np.dtype(bool): t.bool,
+                     np.dtype(np.uint8): t.uint8}[a.dtype]
         if a.dtype == np.uint32:
             a = a.view(np.int32)
         return t.from_numpy(a).to(self.device, dtype=dtype)

@@ -476,6 +476,16 @@ def main():
               "(sharding across GPUs is not built)." % a.gpus, file=sys.stderr)
         sys.exit(2)
 
+    if a.calib_copy_mib:
+        # one device copy of known size: the reference dispatch that HBM byte
+        # counters (FETCH_SIZE / WRITE_SIZE) of this run are calibrated against
+        import torch
+        n = a.calib_copy_mib * (1 << 20) // 8
+        src = torch.ones(n, dtype=torch.float64, device="cuda")
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        del src, dst
     if a.mode == "batch":
         return bench_batch(a)
     if a.mode == "serve":
@@ -486,13 +496,6 @@ def main():
     from vlxmi import pairs as P
     from vlxmi.fock import FockEngine
 
-    if a.calib_copy_mib:
-        n = a.calib_copy_mib * (1 << 20) // 8
-        src = torch.ones(n, dtype=torch.float64, device="cuda")
-        dst = torch.empty_like(src)
-        dst.copy_(src)
-        torch.cuda.synchronize()
-        del src, dst
     mol, D = configs.make_workload(a.workload)
     t0 = time.time()
     eng = FockEngine(mol, fp32_threshold=a.fp32_threshold, mixed_policy=a.mixed_policy)

@@ -1,8 +1,8 @@
 """Continuous-batching server on the GPU.
 
-* Poisson arrivals with two batches in flight: requests admitted while another
-  batch is on the GPU, and each request's J/K equal (bitwise) to the same
-  request served alone.
+* Bursts of arrivals with two batches in flight: requests admitted while
+  another batch is on the GPU, and each request's J/K equal (bitwise) to the
+  same request served alone.
 * The serving workload's generator (molecules of 8-200 atoms at the 8-atom
   workload's number density, 2 s + 1 p per atom, densities perturbed per SCF
   iteration) on a slice with a 128-atom molecule: every mixed result's report
@@ -32,15 +32,18 @@ def dev():
 
 def test_served_equals_alone(dev):
     from vlxmi.batch import BatchEngine, MolEntry, Pack
-    from vlxmi.serve import FockServer, poisson_requests
-    _say("case: serving 6 molecules x 3 SCF iterations, Poisson arrivals, two batches in flight")
+    from vlxmi.serve import FockServer, Request
+    _say("case: serving 6 molecules x 3 SCF iterations in bursts, two batches in flight")
     mols, dens = {}, []
     for m in range(6):
         mol, D = configs.make_batch_molecule(m, lo=4, hi=14)
         mols[m] = mol
         dens.append(D)
-    reqs = poisson_requests(dens, 3, rate=400.0, seed=1,
-                            perturb=lambda D0, m, it: configs.serve_iteration_density(D0, m, it))
+    # every molecule's iteration k arrives together at 30 ms * k: each burst is
+    # more than one batch (max_batch 3), so batches mix molecules and the
+    # second batch of a burst is launched while the first is on the GPU
+    reqs = [Request(rid=len(mols) * it + m, mol_id=m, iteration=it, arrival=0.03 * it,
+                    D=configs.serve_iteration_density(dens[m], m, it)) for it in range(3) for m in range(len(mols))]
     srv = FockServer(mols, dev=dev, hbm_budget=4e9, max_batch=3, precision="mixed", keep_results=True)
     st = srv.serve(reqs)
     _say("  batches %d, max in flight %d, launched while another ran %d"

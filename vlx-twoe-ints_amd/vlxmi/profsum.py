@@ -140,6 +140,48 @@ def counters_markdown(pmc_dir, per_class_path, top=25, copy_bytes=512 << 20):
     return "\n".join(o) + "\n"
 
 
+def trace_by_grid(path, kernels, top=None):
+    """Per (kernel, grid size) durations from a kernel trace
+    (run_kernel_trace.csv): a run that autotunes a kernel's grid launches it
+    at several sizes; the tuned size's average is the one timed runs see."""
+    rows = {}
+    with open(path) as f:
+        for r in csv.DictReader(f):
+            name = _short(r["Kernel_Name"])
+            if name not in kernels:
+                continue
+            g = r.get("Grid_Size_X") or r.get("Grid_Size") or "?"
+            wg = r.get("Workgroup_Size_X") or r.get("Workgroup_Size") or "?"
+            d = rows.setdefault((name, g, wg), [])
+            d.append((int(r["End_Timestamp"]) - int(r["Start_Timestamp"])) * 1e-6)
+    o = ["| kernel | grid (threads) | workgroup | calls | avg ms | min ms | max ms |", "|---|---:|---:|---:|---:|---:|---:|"]
+    for (name, g, wg), v in sorted(rows.items(), key=lambda kv: (kv[0][0], -len(kv[1]))):
+        o.append("| %s | %s | %s | %d | %.3f | %.3f | %.3f |" % (name, g, wg, len(v), sum(v) / len(v), min(v), max(v)))
+    return "\n".join(o) + "\n"
+
+
+def traffic_json(fetch_path, write_path, copy_bytes, command=""):
+    """Per-kernel HBM bytes per dispatch from a FETCH_SIZE pass and a
+    WRITE_SIZE pass of one command, each scaled by what the counter reads for
+    the device copy of ``copy_bytes`` bytes in the same run (bytes per counter
+    unit).  Returns {"calibration": ..., "kernels": {name: {"read", "write",
+    "dispatches"}}}."""
+    kf, kw = copy_calibration(fetch_path, write_path, copy_bytes)
+    out = {}
+    for path, cname, scale, key in ((fetch_path, "FETCH_SIZE", kf, "read"), (write_path, "WRITE_SIZE", kw, "write")):
+        acc = {}
+        for (_, name), v in counters_from(path).items():
+            a = acc.setdefault(name, [0.0, 0])
+            a[0] += v.get(cname, 0.0)
+            a[1] += 1
+        for name, (tot, n) in acc.items():
+            d = out.setdefault(name, {"read": 0.0, "write": 0.0, "dispatches": n})
+            d[key] = tot / n * scale if scale else None
+    return {"command": command, "calibration": {"copy_bytes": copy_bytes, "bytes_per_fetch_unit": kf,
+                                                "bytes_per_write_unit": kw},
+            "kernels": out}
+
+
 def busy_markdown(path, top=30):
     """Per-kernel issue and memory-pipeline figures of one counter pass
     (SQ_WAVES, SQ_WAVE_CYCLES, SQ_INSTS_VALU, SQ_ACTIVE_INST_VALU,
@@ -184,8 +226,18 @@ def main():
     ap.add_argument("--counters", default=None, help="directory holding the mem/fetch/write counter passes")
     ap.add_argument("--per-class", default=None, help="per-class JSON written by bench.py --per-class-out")
     ap.add_argument("--busy", default=None, help="run_counter_collection.csv of one issue/memory counter pass")
+    ap.add_argument("--traffic", nargs=2, metavar=("FETCH_CSV", "WRITE_CSV"), default=None,
+                    help="FETCH_SIZE and WRITE_SIZE passes of one command: per-kernel HBM bytes (JSON)")
+    ap.add_argument("--copy-bytes", type=int, default=512 << 20, help="size of the calibration copy in those runs")
+    ap.add_argument("--trace", default=None, help="run_kernel_trace.csv: per (kernel, grid size) durations")
+    ap.add_argument("--kernels", nargs="*", default=[], help="with --trace: the kernels to list")
     a = ap.parse_args()
-    if a.busy:
+    if a.trace:
+        print(trace_by_grid(a.trace, set(a.kernels)), end="")
+    elif a.traffic:
+        import json
+        print(json.dumps(traffic_json(a.traffic[0], a.traffic[1], a.copy_bytes, a.command), indent=1, sort_keys=True))
+    elif a.busy:
         print(busy_markdown(a.busy, a.top), end="")
     elif a.counters:
         print(counters_markdown(a.counters, a.per_class, a.top), end="")

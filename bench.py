@@ -444,6 +444,57 @@ def bench_serve(a):
     print(json.dumps(out), flush=True)
 
 
+def bench_grad(a):
+    """Two-electron energy gradient at fixed density (the J and K parts,
+    vlxmi.grad) of one molecule (--workload), FP64, beside the same molecule's
+    FP64 J+K build; per-class launch times by HIP events."""
+    import numpy as np
+    import torch
+    from vlxmi import configs
+    from vlxmi.fock import FockEngine
+    from vlxmi.grad import GradientEngine
+    mol, D = configs.make_workload(a.workload)
+    eng = FockEngine(mol)
+    ge = GradientEngine(eng)
+    Dd = eng.dev.tensor(D)
+    torch.cuda.synchronize()
+    _progress("grad: %s lists and device data ready" % a.workload)
+    for _ in range(a.warmup):
+        ge.gradient(Dd)
+        eng.build(Dd)
+        torch.cuda.synchronize()
+        _progress("grad: warm-up gradient and build done")
+    eng.timer = []
+    t0 = time.perf_counter()
+    for _ in range(a.steps):
+        gJ, gK = ge.gradient(Dd)
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    timer, eng.timer = eng.timer, None
+    _progress("grad: gradient %.3f s" % (dt / a.steps))
+    t0 = time.perf_counter()
+    for _ in range(a.steps):
+        eng.build(Dd)
+    torch.cuda.synchronize()
+    dtf = time.perf_counter() - t0
+    per = {}
+    for name, e0, e1 in timer:
+        per.setdefault(name, []).append(e0.elapsed_time(e1) * 1e-3)
+    top = sorted(per, key=lambda k: -sum(per[k]))[:8]
+    gJ, gK = gJ.cpu().numpy(), gK.cpu().numpy()
+    out = {"metric": "two-electron energy gradients/s (J and K parts at fixed density, FP64)",
+           "value": a.steps / dt, "unit": "gradients/s", "n_gpus": 1, "steps": a.steps, "warmup": a.warmup,
+           "ms_per_step": 1e3 * dt / a.steps, "higher_is_better": True, "dtype": "f64", "data": "synthetic",
+           "config": {"workload": "%s: %d atoms, nbf %d" % (a.workload, mol.natoms, mol.nbf)},
+           "fock_build_ms": 1e3 * dtf / a.steps, "gradient_over_fock_build": dt / dtf,
+           "kernel_s_per_gradient": sum(sum(v) for v in per.values()) / a.steps,
+           "top_kernels": [{"name": k, "avg_ms": 1e3 * sum(per[k]) / len(per[k])} for k in top],
+           "max_abs_gJ": float(np.abs(gJ).max()), "max_abs_gK": float(np.abs(gK).max()),
+           "sum_over_atoms_gJ": float(np.abs(gJ.sum(axis=0)).max()),
+           "sum_over_atoms_gK": float(np.abs(gK.sum(axis=0)).max())}
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -460,9 +511,10 @@ def main():
     ap.add_argument("--mixed-policy", choices=("measured", "all"), default="measured",
                     help="mixed precision: launch _mixed kernels only for the classes of "
                          "vlxmi/gen/mixed_classes.json (measured faster) or for every class")
-    ap.add_argument("--mode", choices=("fock", "batch", "serve"), default="batch",
+    ap.add_argument("--mode", choices=("fock", "batch", "serve", "grad"), default="batch",
                     help="batch (default, the headline): 512 molecules per persistent launch vs the single-molecule "
-                         "FP64 path; fock: one molecule's J+K build (--workload); serve: request server")
+                         "FP64 path; fock: one molecule's J+K build (--workload); serve: request server; "
+                         "grad: J/K energy gradient of one molecule (--workload)")
     ap.add_argument("--no-secondary", action="store_true", help="batch: skip the lattice64 FP64 / CPU figures")
     ap.add_argument("--molecules", type=int, default=None, help="batch/serve: number of molecules")
     ap.add_argument("--max-atoms", type=int, default=None, help="batch/serve: largest molecule (atoms)")
@@ -490,6 +542,8 @@ def main():
         return bench_batch(a)
     if a.mode == "serve":
         return bench_serve(a)
+    if a.mode == "grad":
+        return bench_grad(a)
     import numpy as np
     import torch
     from vlxmi import configs, flops

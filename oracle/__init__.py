@@ -49,6 +49,11 @@ def lib():
         L.orc_eri_block.argtypes = [C.c_int] * 4 + [C.c_double, _dp, C.c_double, _dp,
                                                     C.c_double, _dp, C.c_double, _dp, C.c_double, _dp]
         L.orc_eri_block.restype = None
+        L.orc_eri_deriv_block.argtypes = [C.c_int] * 5 + [C.c_double, _dp, C.c_double, _dp,
+                                                          C.c_double, _dp, C.c_double, _dp, C.c_double, _dp]
+        L.orc_eri_deriv_block.restype = None
+        L.orc_coulomb_grad_class.restype = C.c_int
+        L.orc_exchange_grad_class.restype = C.c_int
         L.orc_schwarz.argtypes = [C.c_int, C.c_int, _dp, C.c_uint32, _dp, C.c_uint32,
                                   _up, _up, C.c_uint32, _dp, _dp]
         L.orc_schwarz.restype = None
@@ -70,10 +75,20 @@ def eri_block(la, lb, lc, ld, ai, A, aj, B, ak, Cc, al, D, omega=0.0) -> np.ndar
     quartet (no contraction / normalisation factors); shape (na, nb, nc, nd)."""
     from math import comb
     n = [(l + 1) * (l + 2) // 2 for l in (la, lb, lc, ld)]
-    out = np.zeros(1296)
+    out = np.zeros(4096)
     f = lambda v: np.ascontiguousarray(v, dtype=np.float64)
     lib().orc_eri_block(la, lb, lc, ld, ai, f(A), aj, f(B), ak, f(Cc), al, f(D), omega, out)
     return out[: n[0] * n[1] * n[2] * n[3]].reshape(n)
+
+
+def eri_deriv_block(pos, la, lb, lc, ld, ai, A, aj, B, ak, Cc, al, D, omega=0.0) -> np.ndarray:
+    """d/dR_g of eri_block for R the centre of the function in position ``pos``
+    (0 a, 1 b, 2 c, 3 d); shape (3, na, nb, nc, nd)."""
+    n = [(l + 1) * (l + 2) // 2 for l in (la, lb, lc, ld)]
+    out = np.zeros(3 * 4096)
+    f = lambda v: np.ascontiguousarray(v, dtype=np.float64)
+    lib().orc_eri_deriv_block(pos, la, lb, lc, ld, ai, f(A), aj, f(B), ak, f(Cc), al, f(D), omega, out)
+    return out[: 3 * n[0] * n[1] * n[2] * n[3]].reshape([3] + n)
 
 
 def schwarz(la, lb, info_a, info_b, I, J, S):
@@ -171,6 +186,88 @@ def exchange_class(la, lb, lc, ld, ik_i, ik_k, prim_info, prim_aoinds, prim_coun
     if rc != 0:
         raise ValueError("oracle exchange_class: ik list size is not whole blocks")
     return mat_K
+
+
+def coulomb_grad_class(la, lb, lc, ld, prim_info, prim_count, ab_mat_D, cd_mat_D, ab_mat_Q, cd_mat_Q,
+                       ab_first, ab_second, ab_pair_data, cd_first, cd_second, cd_pair_data,
+                       thr, bra_blocks=None, grad=None, n_eval=None):
+    """Coulomb-energy gradient parts (+=) of one J class, shape (bra blocks, 6):
+    [0:3] the bra A-centre part, [3:6] the B-centre part (orc_coulomb_grad_class)."""
+    arrs = []
+
+    def keep(a, dt):
+        a = np.ascontiguousarray(a, dtype=dt)
+        arrs.append(a)
+        return a
+
+    infos = [keep(prim_info[l] if prim_info[l] is not None else np.zeros((5, 1)), np.float64) for l in range(3)]
+    info_ptrs = (C.c_void_p * 3)(*[i.ctypes.data for i in infos])
+    counts = keep(np.array([prim_count[l] for l in range(3)]), np.uint32)
+    ab_first = keep(ab_first, np.uint32)
+    nab = (la + 1) * (la + 2) // 2 * ((lb + 1) * (lb + 2) // 2)
+    if grad is None:
+        grad = np.zeros((ab_first.size // nab, 6))
+    assert grad.dtype == np.float64 and grad.flags.c_contiguous and grad.size == 6 * (ab_first.size // nab)
+    if bra_blocks is not None:
+        bb = keep(bra_blocks, np.uint32)
+        bptr, nb = _ptr(bb), bb.size
+    else:
+        bptr, nb = None, 0
+    rc = lib().orc_coulomb_grad_class(
+        C.c_int(la), C.c_int(lb), C.c_int(lc), C.c_int(ld), info_ptrs, _ptr(counts),
+        _ptr(keep(ab_mat_D, np.float64)), _ptr(keep(cd_mat_D, np.float64)),
+        _ptr(keep(ab_mat_Q, np.float64)), _ptr(keep(cd_mat_Q, np.float64)),
+        _ptr(ab_first), _ptr(keep(ab_second, np.uint32)), _ptr(keep(ab_pair_data, np.float64)),
+        C.c_uint32(ab_first.size),
+        _ptr(keep(cd_first, np.uint32)), _ptr(keep(cd_second, np.uint32)), _ptr(keep(cd_pair_data, np.float64)),
+        C.c_uint32(np.asarray(cd_first).size),
+        C.c_double(thr), bptr, C.c_uint32(nb), _ptr(grad), _ptr(n_eval) if n_eval is not None else None)
+    if rc != 0:
+        raise ValueError("oracle coulomb_grad_class: list sizes are not whole blocks")
+    return grad
+
+
+def exchange_grad_class(la, lb, lc, ld, ik_i, ik_k, prim_info, prim_aoinds, prim_count, max_D_bd,
+                        D_AO, naos, Q_ab, Q_cd, Dinds_ab, Dinds_cd, displs_ab, displs_cd,
+                        counts_ab, counts_cd, pd_ab, pd_cd, omega, thr, ik_blocks=None, grad=None, n_eval=None):
+    """Exchange-energy gradient parts (+=) of one K class, shape ((i,k) blocks, 6):
+    [0:3] the A-centre part, [3:6] the C-centre part (orc_exchange_grad_class)."""
+    arrs = []
+
+    def keep(a, dt):
+        a = np.ascontiguousarray(a, dtype=dt)
+        arrs.append(a)
+        return a
+
+    infos = [keep(prim_info[l] if prim_info[l] is not None else np.zeros((5, 1)), np.float64) for l in range(3)]
+    aos = [keep(prim_aoinds[l] if prim_aoinds[l] is not None else np.zeros((1, 1)), np.uint32) for l in range(3)]
+    info_ptrs = (C.c_void_p * 3)(*[i.ctypes.data for i in infos])
+    ao_ptrs = (C.c_void_p * 3)(*[i.ctypes.data for i in aos])
+    counts = keep(np.array([prim_count[l] for l in range(3)]), np.uint32)
+    ik_i = keep(ik_i, np.uint32)
+    nac = (la + 1) * (la + 2) // 2 * ((lc + 1) * (lc + 2) // 2)
+    if grad is None:
+        grad = np.zeros((ik_i.size // nac, 6))
+    assert grad.dtype == np.float64 and grad.flags.c_contiguous and grad.size == 6 * (ik_i.size // nac)
+    if ik_blocks is not None:
+        bb = keep(ik_blocks, np.uint32)
+        bptr, nb = _ptr(bb), bb.size
+    else:
+        bptr, nb = None, 0
+    rc = lib().orc_exchange_grad_class(
+        C.c_int(la), C.c_int(lb), C.c_int(lc), C.c_int(ld), _ptr(ik_i), _ptr(keep(ik_k, np.uint32)),
+        C.c_uint32(ik_i.size), info_ptrs, ao_ptrs, _ptr(counts), C.c_double(max_D_bd),
+        _ptr(keep(D_AO, np.float64)), C.c_uint32(naos),
+        _ptr(keep(Q_ab, np.float64)), _ptr(keep(Q_cd, np.float64)),
+        _ptr(keep(Dinds_ab, np.uint32)), _ptr(keep(Dinds_cd, np.uint32)),
+        _ptr(keep(displs_ab, np.uint32)), _ptr(keep(displs_cd, np.uint32)),
+        _ptr(keep(counts_ab, np.uint32)), _ptr(keep(counts_cd, np.uint32)),
+        _ptr(keep(pd_ab, np.float64)), _ptr(keep(pd_cd, np.float64)),
+        C.c_double(omega), C.c_double(thr), bptr, C.c_uint32(nb), _ptr(grad),
+        _ptr(n_eval) if n_eval is not None else None)
+    if rc != 0:
+        raise ValueError("oracle exchange_grad_class: ik list size is not whole blocks")
+    return grad
 
 
 def ao_eri_tensor(mol) -> np.ndarray:

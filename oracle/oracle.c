@@ -40,9 +40,12 @@
 #include <stdlib.h>
 #include <string.h>
 
-#define MAXL_SIDE 4           /* la+lb, lc+ld <= 4 for s,p,d */
-#define NTRI 35               /* triples with |e| <= 4 */
-#define MAXM 9                /* m = 0..8 */
+/* Sizes cover the derivative classes too: one shell raised to f (l = 3) for
+ * the geometric derivatives, so la+lb, lc+ld <= 5 and L <= 9. */
+#define MAXL_SIDE 5           /* la+lb, lc+ld <= 5 */
+#define NTRI 56               /* triples with |e| <= 5 */
+#define MAXM 10               /* m = 0..10 */
+#define NSH 10                /* Cartesian components of one shell, l <= 3 */
 
 static const double ORC_PI = 3.14159265358979323846264338327950288;
 
@@ -201,9 +204,9 @@ void orc_eri_block(int la, int lb, int lc, int ld,
     /* HRR on d (ket): W[e][c][d] for e with la <= |e| <= Le */
     const int nc = ncart_of(lc), nd = ncart_of(ld), na = ncart_of(la), nb = ncart_of(lb);
     /* X[lvl][e][c'][d'] with |c'| from lc .. Lf-lvl, |d'| = lvl */
-    static _Thread_local double Xk[NTRI][NTRI][6]; /* [e][c' triple index][d' pos] current level */
-    static _Thread_local double Xn[NTRI][NTRI][6];
-    static _Thread_local double W[NTRI][6][6];     /* [e][c][d] */
+    static _Thread_local double Xk[NTRI][NTRI][NSH]; /* [e][c' triple index][d' pos] current level */
+    static _Thread_local double Xn[NTRI][NTRI][NSH];
+    static _Thread_local double W[NTRI][NSH][NSH];   /* [e][c][d] */
     for (int ie = 0; ie < NTRI; ++ie) {
         int ne;
         { int cnt = 0; ne = 0; while (cnt + ncart_of(ne) <= ie) { cnt += ncart_of(ne); ++ne; } }
@@ -242,7 +245,7 @@ void orc_eri_block(int la, int lb, int lc, int ld,
         }
     }
     /* HRR on b (bra), per (c,d) */
-    static _Thread_local double Yk[NTRI][6], Yn[NTRI][6];
+    static _Thread_local double Yk[NTRI][NSH], Yn[NTRI][NSH];
     for (int pc = 0; pc < nc; ++pc)
         for (int pd = 0; pd < nd; ++pd) {
             for (int ne = la; ne <= Le; ++ne)
@@ -279,6 +282,68 @@ void orc_eri_block(int la, int lb, int lc, int ld,
                     out[((pa * nb + pb) * nc + pc) * nd + pd] = Yk[ia][pb];
             }
         }
+}
+
+/* ------------------------------------------------------------------------ */
+/* Geometric derivative of one primitive quartet                            */
+/* ------------------------------------------------------------------------ */
+
+static size_t blk_index(const int* p, const int* n) {
+    return (((size_t)p[0] * n[1] + p[1]) * n[2] + p[2]) * n[3] + p[3];
+}
+
+/*
+ * Derivative of one primitive quartet with respect to the centre of the
+ * function in position pos (0: a, 1: b, 2: c, 3: d): Obara-Saika, JCP 84, 3963
+ * (1986), eq. (5), the operator the reference applies symbolically in
+ * eri.py:85-175 (apply_gradient_a..d) and eriwriter.py:72-86:
+ *   d/dR_g ( .. r .. ) = 2 alpha_r ( .. r+1_g .. ) - N_g(r) ( .. r-1_g .. )
+ * with alpha_r the exponent of that function and N_g(r) its exponent of
+ * direction g.  The raised and lowered classes are evaluated by orc_eri_block
+ * with the same exponents and centres (so, like orc_eri_block, without the
+ * pair factors S, which do not depend on the angular momentum).
+ * out[g * n + t], g = x, y, z, t = ((ia*nb + ib)*nc + ic)*nd + id,
+ * n = na nb nc nd.  Each side (la+lb, lc+ld) may be at most 5 after raising.
+ */
+void orc_eri_deriv_block(int pos, int la, int lb, int lc, int ld,
+                         double ai, const double* A, double aj, const double* B,
+                         double ak, const double* C, double al, const double* D,
+                         double omega, double* out)
+{
+    const int l[4] = {la, lb, lc, ld};
+    const double ex[4] = {ai, aj, ak, al};
+    int n[4], lu[4], nu[4], lm[4], nm[4];
+    for (int k = 0; k < 4; ++k) { lu[k] = lm[k] = l[k]; n[k] = ncart_of(l[k]); }
+    lu[pos] += 1;
+    lm[pos] -= 1;
+    for (int k = 0; k < 4; ++k) { nu[k] = ncart_of(lu[k]); nm[k] = lm[k] >= 0 ? ncart_of(lm[k]) : 0; }
+    static _Thread_local double up[4096], dn[4096];
+    orc_eri_block(lu[0], lu[1], lu[2], lu[3], ai, A, aj, B, ak, C, al, D, omega, up);
+    if (l[pos] > 0) orc_eri_block(lm[0], lm[1], lm[2], lm[3], ai, A, aj, B, ak, C, al, D, omega, dn);
+    const size_t ntot = (size_t)n[0] * n[1] * n[2] * n[3];
+    int p[4];
+    for (p[0] = 0; p[0] < n[0]; ++p[0])
+    for (p[1] = 0; p[1] < n[1]; ++p[1])
+    for (p[2] = 0; p[2] < n[2]; ++p[2])
+    for (p[3] = 0; p[3] < n[3]; ++p[3]) {
+        const size_t t = blk_index(p, n);
+        int e[3];
+        tri_comp(l[pos], p[pos], e);
+        for (int g = 0; g < 3; ++g) {
+            int q[4] = {p[0], p[1], p[2], p[3]};
+            int eu[3] = {e[0], e[1], e[2]};
+            eu[g] += 1;
+            q[pos] = tri_pos(eu[0], eu[1], eu[2]);
+            double v = 2.0 * ex[pos] * up[blk_index(q, nu)];
+            if (e[g] > 0) {
+                int em[3] = {e[0], e[1], e[2]};
+                em[g] -= 1;
+                q[pos] = tri_pos(em[0], em[1], em[2]);
+                v -= (double)e[g] * dn[blk_index(q, nm)];
+            }
+            out[(size_t)g * ntot + t] = v;
+        }
+    }
 }
 
 /* ------------------------------------------------------------------------ */
@@ -571,4 +636,212 @@ void orc_ao_eri(int nshell, const int* shl_l, const int* shl_nprim, const double
                     }
         free(blk);
     }
+}
+
+/* ------------------------------------------------------------------------ */
+/* Energy-gradient contributions of the J and K classes                      */
+/* ------------------------------------------------------------------------ */
+
+/*
+ * Coulomb-energy gradient of one J class over the lists of orc_coulomb_class,
+ * with its screening (a component quartet counts iff |Q_ij Q_kl D_kl| > thr,
+ * gen_coulomb.py:209-212).  For E_J = 1/2 sum_ab D_ab J_ab with D symmetric,
+ * the bra-ket and a<->b symmetries of the ordered sum give
+ *   dE_J/dX = sum_{bra entries ab} w_ab D_ab sum_{ket entries cd} w_cd D_cd
+ *             [ dA (ab|cd) if a sits on X  +  dB (ab|cd) if b sits on X ]
+ * with w the J weight (2, or 1 for an entry with first == second of a
+ * same-type list; gen_coulomb.py:276-281) and dA, dB the derivatives of
+ * orc_eri_deriv_block times the pair factors.  ab_mat_D holds the bra
+ * entries' D in the ket role's convention (0 below the diagonal of a diagonal
+ * block).  grad[6 bb + 0..2] += the A-centre part and grad[6 bb + 3..5] +=
+ * the B-centre part of every selected bra block bb.
+ */
+int orc_coulomb_grad_class(int la, int lb, int lc, int ld,
+                           const double* const* prim_info, const uint32_t* prim_count,
+                           const double* ab_mat_D, const double* cd_mat_D,
+                           const double* ab_mat_Q, const double* cd_mat_Q,
+                           const uint32_t* ab_first, const uint32_t* ab_second,
+                           const double* ab_pair_data, uint32_t ab_count,
+                           const uint32_t* cd_first, const uint32_t* cd_second,
+                           const double* cd_pair_data, uint32_t cd_count,
+                           double thr, const uint32_t* bra_blocks, uint32_t n_bra_blocks,
+                           double* grad, unsigned long long* n_eval)
+{
+    unsigned long long nev = 0;
+    const int na = ncart_of(la), nb = ncart_of(lb), nc = ncart_of(lc), nd = ncart_of(ld);
+    const int nab = na * nb, ncd = nc * nd, n = nab * ncd;
+    if (ab_count % nab || cd_count % ncd) return -1;
+    const uint32_t nbra = ab_count / nab, nket = cd_count / ncd;
+    const uint32_t nsel = bra_blocks ? n_bra_blocks : nbra;
+    const int same_ab = (la == lb), same_cd = (lc == ld);
+    double* beta = (double*)malloc(sizeof(double) * (nket ? nket : 1));
+    for (uint32_t kb = 0; kb < nket; ++kb) {
+        double m = 0.0;
+        for (int t = 0; t < ncd; ++t) {
+            double v = fabs(cd_mat_Q[(size_t)kb * ncd + t] * cd_mat_D[(size_t)kb * ncd + t]);
+            if (v > m) m = v;
+        }
+        beta[kb] = m;
+    }
+    #pragma omp parallel reduction(+:nev)
+    {
+        double* dA = (double*)malloc(sizeof(double) * 3 * 1296);
+        double* dB = (double*)malloc(sizeof(double) * 3 * 1296);
+        #pragma omp for schedule(dynamic, 4)
+        for (uint32_t s = 0; s < nsel; ++s) {
+            const uint32_t bb = bra_blocks ? bra_blocks[s] : s;
+            const size_t e0 = (size_t)bb * nab;
+            const uint32_t I = ab_first[e0] / na, J = ab_second[e0] / nb;
+            double ai, aj, A[3], B[3];
+            prim_of(prim_info[la], prim_count[la], I, &ai, A);
+            prim_of(prim_info[lb], prim_count[lb], J, &aj, B);
+            double qmax = 0.0;
+            for (int t = 0; t < nab; ++t) { double q = fabs(ab_mat_Q[e0 + t]); if (q > qmax) qmax = q; }
+            long double acc[6] = {0.0L, 0.0L, 0.0L, 0.0L, 0.0L, 0.0L};
+            for (uint32_t kb = 0; kb < nket; ++kb) {
+                if (qmax * beta[kb] * (1.0 + 1e-13) <= thr) continue;
+                const size_t f0 = (size_t)kb * ncd;
+                const uint32_t K = cd_first[f0] / nc, Lp = cd_second[f0] / nd;
+                double ak, al, C[3], D[3];
+                prim_of(prim_info[lc], prim_count[lc], K, &ak, C);
+                prim_of(prim_info[ld], prim_count[ld], Lp, &al, D);
+                orc_eri_deriv_block(0, la, lb, lc, ld, ai, A, aj, B, ak, C, al, D, 0.0, dA);
+                orc_eri_deriv_block(1, la, lb, lc, ld, ai, A, aj, B, ak, C, al, D, 0.0, dB);
+                ++nev;
+                for (int tab = 0; tab < nab; ++tab) {
+                    const double Qij = ab_mat_Q[e0 + tab];
+                    const double wab = (same_ab && ab_first[e0 + tab] == ab_second[e0 + tab]) ? 1.0 : 2.0;
+                    const double fab = ab_pair_data[e0 + tab] * ab_mat_D[e0 + tab] * wab;
+                    for (int tcd = 0; tcd < ncd; ++tcd) {
+                        const size_t kl = f0 + tcd;
+                        const double Dkl = cd_mat_D[kl];
+                        if (!(fabs(Qij * cd_mat_Q[kl] * Dkl) > thr)) continue;
+                        const double wcd = (same_cd && cd_first[kl] == cd_second[kl]) ? 1.0 : 2.0;
+                        const double f = fab * cd_pair_data[kl] * Dkl * wcd;
+                        const int t = tab * ncd + tcd;
+                        for (int g = 0; g < 3; ++g) {
+                            acc[g] += (long double)(f * dA[g * n + t]);
+                            acc[3 + g] += (long double)(f * dB[g * n + t]);
+                        }
+                    }
+                }
+            }
+            for (int g = 0; g < 6; ++g) grad[6 * (size_t)bb + g] = (double)((long double)grad[6 * (size_t)bb + g] + acc[g]);
+        }
+        free(dA);
+        free(dB);
+    }
+    free(beta);
+    if (n_eval) *n_eval += nev;
+    return 0;
+}
+
+/*
+ * Exchange-energy gradient of one K class over the lists of
+ * orc_exchange_class, with its screening (a component quartet counts iff
+ * |Q_ij Q_kl maxD_bd| > thr, gen_exchange.py:229-232).  For
+ * E_K = 1/2 sum_ac D_ac K_ac with D symmetric, the symmetries of the ordered
+ * sum give
+ *   dE_K/dX = sum_{(I,K) blocks} m_IK sum_{a in I, c in K} D_ac sum_{b,d} D_bd
+ *             [ dA (ab|cd) if a sits on X  +  dC (ab|cd) if c sits on X ]
+ * with m_IK = 1 for a block I = K of a same-type list and 2 otherwise (the
+ * list holds one of (I,K), (K,I)).  grad[6 ib + 0..2] += the A-centre part,
+ * grad[6 ib + 3..5] += the C-centre part of every selected (i,k) block ib.
+ */
+int orc_exchange_grad_class(int la, int lb, int lc, int ld,
+                            const uint32_t* ik_i, const uint32_t* ik_k, uint32_t ik_count,
+                            const double* const* prim_info, const uint32_t* const* prim_aoinds,
+                            const uint32_t* prim_count,
+                            double max_D_bd, const double* D_AO, uint32_t naos,
+                            const double* Q_ab, const double* Q_cd,
+                            const uint32_t* Dinds_ab, const uint32_t* Dinds_cd,
+                            const uint32_t* displs_ab, const uint32_t* displs_cd,
+                            const uint32_t* counts_ab, const uint32_t* counts_cd,
+                            const double* pd_ab, const double* pd_cd,
+                            double omega, double thr,
+                            const uint32_t* ik_blocks, uint32_t n_ik_blocks,
+                            double* grad, unsigned long long* n_eval)
+{
+    unsigned long long nev = 0;
+    const int na = ncart_of(la), nb = ncart_of(lb), nc = ncart_of(lc), nd = ncart_of(ld);
+    const int nac = na * nc, n = na * nb * nc * nd;
+    if (ik_count % nac) return -1;
+    const uint32_t nblk = ik_count / nac;
+    const uint32_t nsel = ik_blocks ? n_ik_blocks : nblk;
+    #pragma omp parallel reduction(+:nev)
+    {
+        double* dA = (double*)malloc(sizeof(double) * 3 * 1296);
+        double* dC = (double*)malloc(sizeof(double) * 3 * 1296);
+        #pragma omp for schedule(dynamic, 2)
+        for (uint32_t s = 0; s < nsel; ++s) {
+            const uint32_t ib = ik_blocks ? ik_blocks[s] : s;
+            const size_t e0 = (size_t)ib * nac;
+            const uint32_t I = ik_i[e0] / na, K = ik_k[e0] / nc;
+            double ai, ak, A[3], C[3];
+            prim_of(prim_info[la], prim_count[la], I, &ai, A);
+            prim_of(prim_info[lc], prim_count[lc], K, &ak, C);
+            const double mIK = (la == lc && I == K) ? 1.0 : 2.0;
+            double Dac[36];
+            for (int ca = 0; ca < na; ++ca)
+                for (int cc = 0; cc < nc; ++cc) {
+                    const uint32_t ao_a = prim_aoinds[la][I + prim_count[la] * ca];
+                    const uint32_t ao_c = prim_aoinds[lc][K + prim_count[lc] * cc];
+                    Dac[ca * nc + cc] = D_AO[(size_t)ao_a * naos + ao_c];
+                }
+            const uint32_t nJ = counts_ab[I * na] / nb, nL = counts_cd[K * nc] / nd;
+            long double acc[6] = {0.0L, 0.0L, 0.0L, 0.0L, 0.0L, 0.0L};
+            for (uint32_t jb = 0; jb < nJ; ++jb) {
+                const uint32_t J = Dinds_ab[displs_ab[I * na] + jb * nb] / nb;
+                double aj, B[3];
+                prim_of(prim_info[lb], prim_count[lb], J, &aj, B);
+                double qij_max = 0.0;
+                for (int ca = 0; ca < na; ++ca)
+                    for (int cb = 0; cb < nb; ++cb) {
+                        double q = fabs(Q_ab[displs_ab[I * na + ca] + jb * nb + cb]);
+                        if (q > qij_max) qij_max = q;
+                    }
+                for (uint32_t lbk = 0; lbk < nL; ++lbk) {
+                    const uint32_t Lp = Dinds_cd[displs_cd[K * nc] + lbk * nd] / nd;
+                    double qkl_max = 0.0;
+                    for (int cc = 0; cc < nc; ++cc)
+                        for (int cd = 0; cd < nd; ++cd) {
+                            double q = fabs(Q_cd[displs_cd[K * nc + cc] + lbk * nd + cd]);
+                            if (q > qkl_max) qkl_max = q;
+                        }
+                    if (qij_max * qkl_max * max_D_bd * (1.0 + 1e-13) <= thr) continue;
+                    double al, Dc[3];
+                    prim_of(prim_info[ld], prim_count[ld], Lp, &al, Dc);
+                    orc_eri_deriv_block(0, la, lb, lc, ld, ai, A, aj, B, ak, C, al, Dc, omega, dA);
+                    orc_eri_deriv_block(2, la, lb, lc, ld, ai, A, aj, B, ak, C, al, Dc, omega, dC);
+                    ++nev;
+                    for (int ca = 0; ca < na; ++ca)
+                        for (int cb = 0; cb < nb; ++cb) {
+                            const size_t pj = displs_ab[I * na + ca] + jb * nb + cb;
+                            const uint32_t jprim = Dinds_ab[pj];
+                            const uint32_t jcg = prim_aoinds[lb][(jprim / nb) + prim_count[lb] * (jprim % nb)];
+                            const double Qij = Q_ab[pj], Sij = pd_ab[pj];
+                            for (int cc = 0; cc < nc; ++cc)
+                                for (int cd = 0; cd < nd; ++cd) {
+                                    const size_t pl = displs_cd[K * nc + cc] + lbk * nd + cd;
+                                    if (!(fabs(Qij * Q_cd[pl] * max_D_bd) > thr)) continue;
+                                    const uint32_t lprim = Dinds_cd[pl];
+                                    const uint32_t lcg = prim_aoinds[ld][(lprim / nd) + prim_count[ld] * (lprim % nd)];
+                                    const double f = mIK * Dac[ca * nc + cc] * Sij * pd_cd[pl]
+                                                     * D_AO[(size_t)jcg * naos + lcg];
+                                    const int t = ((ca * nb + cb) * nc + cc) * nd + cd;
+                                    for (int g = 0; g < 3; ++g) {
+                                        acc[g] += (long double)(f * dA[g * n + t]);
+                                        acc[3 + g] += (long double)(f * dC[g * n + t]);
+                                    }
+                                }
+                        }
+                }
+            }
+            for (int g = 0; g < 6; ++g) grad[6 * (size_t)ib + g] = (double)((long double)grad[6 * (size_t)ib + g] + acc[g]);
+        }
+        free(dA);
+        free(dC);
+    }
+    if (n_eval) *n_eval += nev;
+    return 0;
 }

@@ -90,3 +90,65 @@ def oracle_fock(fl, D_cart, mode="component"):
     mJ = oracle_matJ(fl, D_cart, mode)
     mK = oracle_matK(fl, D_cart, mode)
     return P.assemble_J(fl, mJ), P.assemble_K(fl, mK)
+
+
+def oracle_gradJ(fl, D_cart, bra_blocks=None, classes=None, n_eval=None):
+    """Per bra class the oracle's Coulomb-gradient rows [blocks, 6] (A-centre,
+    B-centre parts), summed over ket classes (all blocks, or the slice
+    ``bra_blocks[ab]`` in slice order)."""
+    out = {}
+    infos = [t.info if t is not None else None for t in fl.prims]
+    counts = [t.count if t is not None else 0 for t in fl.prims]
+    for ab, cd in (classes or P.j_classes()):
+        if ab not in fl.jlists or cd not in fl.jlists:
+            continue
+        bra, ket = fl.jlists[ab], fl.jlists[cd]
+        bD, kD = bra.mat_D(D_cart), ket.mat_D(D_cart)
+        sel = None if bra_blocks is None else np.asarray(bra_blocks[ab], dtype=np.uint32)
+        if ab not in out:
+            out[ab] = np.zeros((bra.nblk, 6))
+        oracle.coulomb_grad_class(bra.la, bra.lb, ket.la, ket.lb, infos, counts, bD, kD, bra.mat_Q, ket.mat_Q,
+                                  bra.first, bra.second, bra.pair_data, ket.first, ket.second, ket.pair_data,
+                                  fl.thr, bra_blocks=sel, grad=out[ab], n_eval=n_eval)
+    if bra_blocks is not None:
+        out = {ab: v[np.asarray(bra_blocks[ab], dtype=np.int64)] for ab, v in out.items()}
+    return out
+
+
+def oracle_gradK(fl, D_cart, ik_blocks=None, classes=None, omega=0.0, n_eval=None):
+    """Per (a,c) class the oracle's exchange-gradient rows [blocks, 6]
+    (A-centre, C-centre parts), summed over (b,d)."""
+    out = {}
+    infos = [t.info if t is not None else None for t in fl.prims]
+    aos = [t.aoinds if t is not None else None for t in fl.prims]
+    counts = [t.count if t is not None else 0 for t in fl.prims]
+    for cls in (classes or P.k_classes()):
+        a, b, c, d = cls
+        ac, ab, cd = a + c, a + b, c + d
+        if ac not in fl.iklists or ab not in fl.kcsr or cd not in fl.kcsr:
+            continue
+        ik = fl.iklists[ac]
+        la, lb, lc, ld = P.LVAL[a], P.LVAL[b], P.LVAL[c], P.LVAL[d]
+        mx = P.max_abs_D_block(D_cart, fl.prims, lb, ld)
+        sel = None if ik_blocks is None else np.asarray(ik_blocks[ac], dtype=np.uint32)
+        if ac not in out:
+            out[ac] = np.zeros((ik.I.size, 6))
+        A, Cc = fl.kcsr[ab], fl.kcsr[cd]
+        oracle.exchange_grad_class(la, lb, lc, ld, ik.i, ik.k, infos, aos, counts, mx, D_cart, fl.naos,
+                                   A.Q, Cc.Q, A.D_inds, Cc.D_inds, A.displs, Cc.displs, A.counts, Cc.counts,
+                                   A.pair_data, Cc.pair_data, omega, fl.thr, ik_blocks=sel, grad=out[ac],
+                                   n_eval=n_eval)
+    if ik_blocks is not None:
+        out = {ac: v[np.asarray(ik_blocks[ac], dtype=np.int64)] for ac, v in out.items()}
+    return out
+
+
+def oracle_gradient(fl, D_cart, omega=0.0):
+    """Per-atom (gJ, gK) of E_J = 1/2 tr(D J), E_K = 1/2 tr(D K) by the oracle."""
+    from vlxmi import grad as G
+    pa = G.prim_atoms(fl)
+    rJ, rK = oracle_gradJ(fl, D_cart), oracle_gradK(fl, D_cart, omega=omega)
+    n = fl.mol.natoms
+    gJ = G.sum_rows(n, [(rJ[ab],) + G.j_block_atoms(fl, ab, pa) for ab in rJ])
+    gK = G.sum_rows(n, [(rK[ac],) + G.k_block_atoms(fl, ac, pa) for ac in rK])
+    return gJ, gK

@@ -174,7 +174,8 @@ class FockEngine:
         return (0.5 * (Dc + Dc.t())).contiguous()
 
     def prepare_kets(self, Dc):
-        """mat_D per J list and the ket-ordered copies (device)."""
+        """mat_D per J list and the ket-ordered copies (device); ``mat_D_bra``
+        keeps mat_D in the resident (bra) order."""
         t = self.torch
         kets = {}
         for pc, j in self.jl.items():
@@ -185,7 +186,7 @@ class FockEngine:
             ent = (order[:, None] * nab + t.arange(nab, device=Dc.device)[None, :]).reshape(-1)
             kets[pc] = dict(first=j["first"][ent].contiguous(), second=j["second"][ent].contiguous(),
                             pair_data=j["pair_data"][ent].contiguous(), mat_Q=j["mat_Q"][ent].contiguous(),
-                            mat_D=mD[ent].contiguous(), count=j["count"])
+                            mat_D=mD[ent].contiguous(), count=j["count"], mat_D_bra=mD)
         return kets
 
     def _ev_start(self):

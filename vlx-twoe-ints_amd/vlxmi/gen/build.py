@@ -34,6 +34,7 @@ import sys
 import time
 
 from . import emit
+from . import emit_grad
 from ..pairs import PAIR_CLASSES, j_classes, k_classes
 
 PKG = os.path.dirname(os.path.dirname(os.path.dirname(os.path.abspath(__file__))))   # vlx-twoe-ints_amd
@@ -139,6 +140,10 @@ def sources(tiles=None, layouts=None):
             out[fn] = inc + emit.emit_k(cls, kt=tiles.get(fn), layout=layouts.get(cls, "reg"),
                                         dorder=dorders.get(cls, "jl"), lds_boys=cls in blds[fam],
                                         wpg=int(wpg[fam].get(cls, 1)), **kw)
+    for ab, cd in j_classes():
+        out["jg_%s.hip" % j_name(ab, cd)] = inc + emit_grad.emit_jgrad(j_name(ab, cd))
+    for cls in k_classes():
+        out["kg_%s.hip" % cls] = inc + emit_grad.emit_kgrad(cls)
     for pc in PAIR_CLASSES:
         out["schwarz_%s.hip" % pc] = inc + emit.emit_schwarz(pc)
     out["jdispatch.hip"] = inc + jdispatch_text()
@@ -218,6 +223,18 @@ def device_header_text():
         o.append("__global__ void computeCoulombFock%s(%s);" % (cls.upper(), ", ".join(emit._strip_restrict(emit.j_signature(cls)))))
     for cls in k_classes():
         o.append("__global__ void computeExchangeFock%s(%s);" % (cls.upper(), ", ".join(emit._strip_restrict(emit.k_signature(cls)))))
+    o += ["",
+          "/* Energy-gradient kernels (no counterpart kernel in the reference, which carries the",
+          " * derivative operator only symbolically, eri.py:85-175):",
+          " *   computeCoulombGradient{AB}{CD}: blockDim 64, gridDim (ceil(bra blocks / 64), ket chunks);",
+          " *     chunk y stores the six sums of bra block bb at grad[(y * nbra + bb) * 6 + 0..5];",
+          " *   computeExchangeGradient{A}{B}{C}{D}: blockDim 64, gridDim = (i,k) blocks; adds six sums",
+          " *     per block to grad[blk * 6 + 0..5]. */"]
+    for ab, cd in j_classes():
+        cls = j_name(ab, cd)
+        o.append("__global__ void computeCoulombGradient%s(%s);" % (cls.upper(), ", ".join(emit._strip_restrict(emit_grad.jg_signature(cls)))))
+    for cls in k_classes():
+        o.append("__global__ void computeExchangeGradient%s(%s);" % (cls.upper(), ", ".join(emit._strip_restrict(emit_grad.kg_signature(cls)))))
     o += ["", "#endif", ""]
     return "\n".join(o)
 
@@ -348,6 +365,25 @@ def header_text():
             for k in range(n):
                 o.append("int vlxmi_launch_%s%d(%s);" % (base, k, ", ".join(sig)))
     o.append("")
+    o.append("/* Energy gradients at fixed density (no counterpart kernel in the reference; the derivative")
+    o.append(" * operator is its Obara-Saika eq. 5, eri.py:85-175, write_eri_code(gradient=...) at")
+    o.append(" * eriwriter.py:72-86).  Same lists and screening as the Fock launchers.")
+    o.append(" * Coulomb: grad_ab[bb*6 + 0..5] += the A-centre (x,y,z) and B-centre parts of d/dX of")
+    o.append(" *   E_J = 1/2 tr(D J) for every bra block bb; ab_mat_D_local = the bra entries' D (0 below the")
+    o.append(" *   diagonal of a diagonal block, like the ket's); ket_split as for the Fock launchers (the")
+    o.append(" *   chunks are added in a fixed order).")
+    o.append(" * Exchange: grad_ac[blk*6 + 0..5] += the A-centre and C-centre parts of d/dX of")
+    o.append(" *   E_K = 1/2 tr(D K) for every (i,k) block (factor 2 for blocks standing for (I,K) and (K,I)).")
+    o.append(" * D must be symmetric.  Summing the rows into atoms: vlxmi/grad.py. */")
+    for ab, cd in j_classes():
+        cls = j_name(ab, cd)
+        sig = emit._strip_restrict(emit_grad.jg_signature(cls))
+        o.append("int vlxmi_launch_computeCoulombGradient%s(%s,\n    uint32_t ket_split, void* stream);" % (cls.upper(), ",\n    ".join(sig)))
+        o.append("")
+    for cls in k_classes():
+        sig = emit._strip_restrict(emit_grad.kg_signature(cls))
+        o.append("int vlxmi_launch_computeExchangeGradient%s(%s,\n    void* stream);" % (cls.upper(), ",\n    ".join(sig)))
+        o.append("")
     for pc in PAIR_CLASSES:
         o.append("/* Schwarz factors sqrt|(ab|ab)| per component pair of primitive-pair blocks (the")
         o.append(" * reference's mat_Q / Q_K inputs, computed by VeloxChem's host there). */")

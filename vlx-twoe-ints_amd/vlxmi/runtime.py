@@ -81,6 +81,15 @@ def lib():
                     fn = getattr(L, "vlxmi_launch_computeExchangeFock%s%d" % (cls.upper(), k))
                     fn.argtypes = [_ctype_of(s) for s in emit.k_signature(cls)] + [C.c_void_p]
                 fn.restype = C.c_int
+    from .gen import emit_grad
+    for ab, cd in j_classes():
+        fn = getattr(L, "vlxmi_launch_computeCoulombGradient" + (ab + cd).upper())
+        fn.argtypes = [_ctype_of(s) for s in emit_grad.jg_signature(ab + cd)] + [C.c_uint32, C.c_void_p]
+        fn.restype = C.c_int
+    for cls in k_classes():
+        fn = getattr(L, "vlxmi_launch_computeExchangeGradient" + cls.upper())
+        fn.argtypes = [_ctype_of(s) for s in emit_grad.kg_signature(cls)] + [C.c_void_p]
+        fn.restype = C.c_int
     for pc in PAIR_CLASSES:
         fn = getattr(L, "vlxmi_schwarz_" + pc)
         fn.argtypes = [C.c_void_p, C.c_uint32, C.c_void_p, C.c_uint32, C.c_void_p, C.c_void_p, C.c_void_p,
@@ -145,6 +154,8 @@ def exported_symbols():
         names += ["vlxmi_launch_computeCoulombFock" + (ab + cd).upper() + "_batched" + mixed for ab, cd in j_classes()]
         names += ["vlxmi_launch_computeExchangeFock" + c.upper() + "_batched" + mixed for c in k_classes()]
     names += ["vlxmi_schwarz_" + pc for pc in PAIR_CLASSES]
+    names += ["vlxmi_launch_computeCoulombGradient" + (ab + cd).upper() for ab, cd in j_classes()]
+    names += ["vlxmi_launch_computeExchangeGradient" + c.upper() for c in k_classes()]
     from .gen.build import SPLIT_SERIES
     for kind, series in SPLIT_SERIES.items():
         base = "vlxmi_launch_computeCoulombFock" if kind == "J" else "vlxmi_launch_computeExchangeFock"

@@ -26,7 +26,7 @@ Screening is the Fock kernels': a component quartet counts iff
 |Q_ij Q_kl D_kl| > thr (J, gen_coulomb.py:209-212) or |Q_ij Q_kl maxD_bd| > thr
 (K, gen_exchange.py:229-232).  The raised and lowered integrals come from the
 same concrete-component DAG as the Fock kernels (one more Boys order), in
-passes over the ket-first-index components when the recursion does not fit the
+passes over groups of ket component pairs when the recursion does not fit the
 registers.  FP64 throughout.
 """
 
@@ -76,13 +76,14 @@ def _deriv_terms(dag, pos, e, fixed, g, two_exp):
 
 
 def _grad_uses(la, lb, lc, ld, group, centres):
-    """The X() calls of one pass (ket-first-index components ``group``)."""
-    for ic in group:
-        cc = tri(lc)[ic]
+    """The component quartets of one pass (ket component pairs ``group``,
+    t = ic * nd + id)."""
+    nd = (ld + 1) * (ld + 2) // 2
+    for t in group:
+        cc, dd = tri(lc)[t // nd], tri(ld)[t % nd]
         for aa in tri(la):
             for bb in tri(lb):
-                for dd in tri(ld):
-                    yield (aa, bb, cc, dd)
+                yield (aa, bb, cc, dd)
 
 
 def _live_estimate_grad(la, lb, lc, ld, group, centres):
@@ -114,19 +115,23 @@ _PLAN = {}
 
 
 def plan_grad_slices(la, lb, lc, ld, kind, budget=REG_BUDGET):
-    """Fewest passes over the ket-first-index components whose live recursion
-    values plus the per-lane persistent values fit ``budget`` doubles."""
+    """Fewest passes over the ket component pairs (t = ic * nd + id, in
+    contiguous equal groups) whose live recursion values plus the per-lane
+    persistent values fit ``budget`` doubles; one pair per pass when none
+    fits.  Passes that fit the registers also keep the compiler's register
+    allocation (the bulk of the compile time of these kernels) fast."""
     key = (la, lb, lc, ld, kind, budget)
     if key in _PLAN:
         return _PLAN[key]
-    na, nb, nc = [(l + 1) * (l + 2) // 2 for l in (la, lb, lc)]
+    na, nb, nc, nd = [(l + 1) * (l + 2) // 2 for l in (la, lb, lc, ld)]
+    ncd = nc * nd
     centres = (0, 1) if kind == "J" else (0, 2)
     pers = 2 * na * nb + 6 + 40 + la + lb + lc + ld + 1
-    best = [[i] for i in range(nc)]
-    for nsl in range(1, nc + 1):
-        if nc % nsl:
+    best = [[t] for t in range(ncd)]
+    for nsl in range(1, ncd + 1):
+        if ncd % nsl:
             continue
-        size = nc // nsl
+        size = ncd // nsl
         groups = [list(range(k * size, (k + 1) * size)) for k in range(nsl)]
         if max(_live_estimate_grad(la, lb, lc, ld, g_, centres) for g_ in groups) + pers <= budget:
             best = groups
@@ -247,21 +252,20 @@ def emit_jgrad(cls):
                 lines += [ind + x for x in launder(L)]
             dag = QuartetDAG(la, lb, lc, ld, prefix="q%d_" % isl, rule_vrr="first", rule_hrr="first")
             dig = []
-            for ic in group:
-                cc = tri(lc)[ic]
-                for idd, dd in enumerate(tri(ld)):
-                    tcd = ic * nd + idd
-                    dig.append("const double Dp%d = R[%d], Qd%d = R[%d];" % (tcd, KREC0 + tcd, tcd, KREC0 + ncd + tcd))
-                    for ia, aa in enumerate(tri(la)):
-                        for ib, bbx in enumerate(tri(lb)):
-                            tab = ia * nb + ib
-                            comp = (aa, bbx, cc, dd)
-                            cf = "cf%d_%d" % (tab, tcd)
-                            dag.lines.append("const double %s = (Qd%d > tau[%d]) ? (wab[%d] * Dp%d) : 0.0;"
-                                             % (cf, tcd, tab, tab, tcd))
-                            for g in range(3):
-                                dag.lines.append("gacc[%d] += %s * (%s);" % (g, cf, _deriv_terms(dag, 0, aa, comp, g, "two_ai")))
-                                dag.lines.append("gacc[%d] += %s * (%s);" % (3 + g, cf, _deriv_terms(dag, 1, bbx, comp, g, "two_aj")))
+            for tcd in group:
+                ic, idd = divmod(tcd, nd)
+                cc, dd = tri(lc)[ic], tri(ld)[idd]
+                dig.append("const double Dp%d = R[%d], Qd%d = R[%d];" % (tcd, KREC0 + tcd, tcd, KREC0 + ncd + tcd))
+                for ia, aa in enumerate(tri(la)):
+                    for ib, bbx in enumerate(tri(lb)):
+                        tab = ia * nb + ib
+                        comp = (aa, bbx, cc, dd)
+                        cf = "cf%d_%d" % (tab, tcd)
+                        dag.lines.append("const double %s = (Qd%d > tau[%d]) ? (wab[%d] * Dp%d) : 0.0;"
+                                         % (cf, tcd, tab, tab, tcd))
+                        for g in range(3):
+                            dag.lines.append("gacc[%d] += %s * (%s);" % (g, cf, _deriv_terms(dag, 0, aa, comp, g, "two_ai")))
+                            dag.lines.append("gacc[%d] += %s * (%s);" % (3 + g, cf, _deriv_terms(dag, 1, bbx, comp, g, "two_aj")))
             lines += [ind + x for x in dig]
             lines += [ind + x for x in _dag_lines(dag)]
         return lines
@@ -458,22 +462,21 @@ def emit_kgrad(cls):
                 lines += [ind + x for x in launder(L)]
             dag = QuartetDAG(la, lb, lc, ld, prefix="q%d_" % isl, rule_vrr="first", rule_hrr="first")
             dig = []
-            for ic in group:
-                cc = tri(lc)[ic]
-                for idd, dd in enumerate(tri(ld)):
-                    tcd = ic * nd + idd
-                    dig.append("const double Sc%d = R[%d], Qk%d = R[%d];" % (tcd, KREC0 + tcd, tcd, KREC0 + ncd + tcd))
-                    for ia, aa in enumerate(tri(la)):
-                        for ib, bbx in enumerate(tri(lb)):
-                            tab = ia * nb + ib
-                            tac = ia * nc + ic
-                            comp = (aa, bbx, cc, dd)
-                            cf = "cf%d_%d" % (tab, tcd)
-                            dag.lines.append("const double %s = (Qk%d > tau[%d]) ? (Dac[%d] * Sab[%d] * Sc%d * Dj%d_%d) : 0.0;"
-                                             % (cf, tcd, tab, tac, tab, tcd, ib, idd))
-                            for g in range(3):
-                                dag.lines.append("gacc[%d] += %s * (%s);" % (g, cf, _deriv_terms(dag, 0, aa, comp, g, "two_ai")))
-                                dag.lines.append("gacc[%d] += %s * (%s);" % (3 + g, cf, _deriv_terms(dag, 2, cc, comp, g, "two_ak")))
+            for tcd in group:
+                ic, idd = divmod(tcd, nd)
+                cc, dd = tri(lc)[ic], tri(ld)[idd]
+                dig.append("const double Sc%d = R[%d], Qk%d = R[%d];" % (tcd, KREC0 + tcd, tcd, KREC0 + ncd + tcd))
+                for ia, aa in enumerate(tri(la)):
+                    for ib, bbx in enumerate(tri(lb)):
+                        tab = ia * nb + ib
+                        tac = ia * nc + ic
+                        comp = (aa, bbx, cc, dd)
+                        cf = "cf%d_%d" % (tab, tcd)
+                        dag.lines.append("const double %s = (Qk%d > tau[%d]) ? (Dac[%d] * Sab[%d] * Sc%d * Dj%d_%d) : 0.0;"
+                                         % (cf, tcd, tab, tac, tab, tcd, ib, idd))
+                        for g in range(3):
+                            dag.lines.append("gacc[%d] += %s * (%s);" % (g, cf, _deriv_terms(dag, 0, aa, comp, g, "two_ai")))
+                            dag.lines.append("gacc[%d] += %s * (%s);" % (3 + g, cf, _deriv_terms(dag, 2, cc, comp, g, "two_ak")))
             lines += [ind + x for x in dig]
             lines += [ind + x for x in _dag_lines(dag)]
         return lines

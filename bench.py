@@ -267,6 +267,8 @@ def bench_batch(a):
         res[prec] = dict(dt=dt, J=J, K=K, per=per, counts=counts, tuned=len(tune))
     # plain FP64 path, one molecule at a time (the drop-in per-class launches)
     engs = [FockEngine(mol, dev=dev, lists=e.fl) for (mol, _), e in zip(mols, entries)]
+    for e in engs:
+        e.concurrent = False       # the plain path: the class launchers in order on one stream
     for m in range(nmol):
         engs[m].build(dens[m])
     torch.cuda.synchronize()
@@ -481,6 +483,9 @@ def bench_grad(a):
     for name, e0, e1 in timer:
         per.setdefault(name, []).append(e0.elapsed_time(e1) * 1e-3)
     top = sorted(per, key=lambda k: -sum(per[k]))[:8]
+    if a.per_class_out:
+        with open(a.per_class_out, "w") as f:
+            json.dump({k: {"avg_s": sum(v) / len(v)} for k, v in per.items()}, f, indent=1, sort_keys=True)
     gJ, gK = gJ.cpu().numpy(), gK.cpu().numpy()
     out = {"metric": "two-electron energy gradients/s (J and K parts at fixed density, FP64)",
            "value": a.steps / dt, "unit": "gradients/s", "n_gpus": 1, "steps": a.steps, "warmup": a.warmup,

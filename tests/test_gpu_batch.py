@@ -5,6 +5,8 @@
 * a molecule gives the same bits alone and in the mix;
 * a captured HIP graph replayed several times (with eager work and syncs in
   between) reproduces the eager result every time;
+* the exchange chain on its own stream beside the Coulomb chain gives the same
+  bits as one stream, eagerly and in graph replays;
 * the autotuner returns a configuration per class and results stay identical.
 """
 
@@ -91,6 +93,29 @@ def test_graph_replay_matches_eager(dev, mols, pack):
         for m in range(len(dens)):
             assert (Jg[m] - Je[m]).abs().max().item() <= TOL
             assert (Kg[m] - Ke[m]).abs().max().item() <= TOL
+
+
+def test_concurrent_chains_bitwise(dev, mols, pack):
+    from vlxmi.batch import BatchEngine
+    entries, pk = pack
+    _say("case: exchange launches on a second stream vs one stream, eager and 3 graph replays (bitwise)")
+    dens = [D for _, D in mols]
+    seq = BatchEngine(dev)
+    seq.concurrent = False
+    Js, Ks = seq.build(pk, dens)
+    con = BatchEngine(dev)
+    assert con.concurrent
+    Jc, Kc = con.build(pk, dens)
+    for m in range(len(dens)):
+        assert bool((Jc[m] == Js[m]).all()) and bool((Kc[m] == Ks[m]).all()), m
+    replay = con.capture(pk, dens)
+    for it in range(3):
+        Je, Ke = seq.build(pk, dens)                 # eager work between replays
+        dev.torch.cuda.synchronize()
+        Jg, Kg = replay(dens)
+        dev.torch.cuda.synchronize()
+        for m in range(len(dens)):
+            assert bool((Jg[m] == Js[m]).all()) and bool((Kg[m] == Ks[m]).all()), (it, m)
 
 
 def test_autotune_keeps_results(dev, mols, pack):

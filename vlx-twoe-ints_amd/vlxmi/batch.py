@@ -12,6 +12,11 @@ pack, and returns each molecule's J and K.  Results of a molecule do not depend
 on the other molecules of the pack (same work items, same order, no atomics),
 so a molecule gives the same bits alone and in any mix.
 
+The exchange launches run on a second stream beside the Coulomb launches (a
+fork/join of two chains that write different buffers), so latency-bound
+Coulomb kernels and gather-bound exchange kernels share the CUs; eager builds
+timed per launch (``timer`` set) stay on one stream.
+
 ``BatchEngine.capture(pack)`` records the whole build in a HIP graph (torch
 CUDA-graph API over the HIP runtime) with static density/output buffers;
 ``replay`` refills the density slots and replays.
@@ -375,6 +380,10 @@ class BatchEngine:
         # which classes launch their _mixed kernel in mixed precision (vlxmi.policy)
         self.mixed_policy = mixed_policy
         self.timer = None
+        # run the exchange launches on a second stream beside the Coulomb ones
+        # (they write different buffers); eager timed builds stay on one stream
+        self.concurrent = True
+        self._kstreams = {}
 
     # ------------------------------------------------------------------ helpers
     def density_flat(self, pack: Pack, densities):
@@ -441,12 +450,21 @@ class BatchEngine:
         (safe to capture in a HIP graph)."""
         from .assemble import zero_
         L = lib()
-        Dflat = pack.Dflat
         kets = pack.kets
-        matJ, matK = pack.matJ, pack.matK
-        maxD = pack.maxD
+        matJ = pack.matJ
         for buf in (pack.jbuf, pack.kbuf, pack.Jflat, pack.Kflat, pack.qcount):
             zero_(self.dev, buf)
+        t = self.torch
+        cur = t.cuda.current_stream(self.dev.device)
+        side = None
+        if self.concurrent and self.timer is None:
+            # fork: the exchange chain waits for the zeroing above, then runs
+            # beside the Coulomb chain; the join below orders the rest
+            key = (self.dev.device, cur.cuda_stream)
+            if key not in self._kstreams:
+                self._kstreams[key] = t.cuda.Stream(device=self.dev.device)
+            side = self._kstreams[key]
+            side.wait_stream(cur)
         st = self.dev.stream_ptr()
         for (ab, cd), (items, cost) in pack.jwork.items():
             bra, ket = pack.jl[ab], kets[cd]
@@ -469,6 +487,20 @@ class BatchEngine:
             args += [tun.get("wg", 0), st]
             name = "computeCoulombFock" + cls.upper() + "_batched" + ("_mixed" if mixed else "")
             self._ev(name, getattr(L, "vlxmi_launch_" + name), *args)
+        if side is not None:
+            with t.cuda.stream(side):
+                self._launch_k(pack, L, side.cuda_stream)
+            cur.wait_stream(side)
+        else:
+            self._launch_k(pack, L, st)
+        pack.jasm.run(pack.jbuf, pack.Jflat)
+        pack.kasm.run(pack.kbuf, pack.Kflat)
+
+    def _launch_k(self, pack, L, st):
+        """The exchange class launches of ``launch`` on stream ``st``."""
+        Dflat = pack.Dflat
+        matK = pack.matK
+        maxD = pack.maxD
         for cls in P.k_classes():
             a, b, c, d = cls
             ac, ab, cd, bd = a + c, a + b, c + d, b + d
@@ -498,8 +530,6 @@ class BatchEngine:
             args += [tun.get("wg", 0), st]
             name = "computeExchangeFock" + cls.upper() + "_batched" + ("_mixed" if mixed else "")
             self._ev(name, getattr(L, "vlxmi_launch_" + name), *args)
-        pack.jasm.run(pack.jbuf, pack.Jflat)
-        pack.kasm.run(pack.kbuf, pack.Kflat)
 
     def quartet_counts(self, pack: Pack):
         """{launch name ("J:abcd" / "K:abcd"): (FP64, FP32) primitive quartets}

@@ -48,6 +48,10 @@ class FockEngine:
         self.target_waves = target_waves
         self.timer = None          # list of (kernel name, start event, end event) when set
         self.progress = None       # progress(msg): called after each class launch has finished (synchronizes)
+        # exchange launches on a second stream beside the Coulomb launches (they
+        # write different buffers); builds timed per launch stay on one stream
+        self.concurrent = True
+        self._kstreams = {}
         # "fp64": plain path; "mixed": quartets whose Schwarz x |D| bound is below
         # fp32_threshold run the VRR/HRR in FP32 (Boys, prefactors, sums in FP64)
         assert precision in ("fp64", "mixed")
@@ -344,6 +348,28 @@ class FockEngine:
                                            self.thr / maxD, out.data_ptr(), self.dev.stream_ptr()), "count_k_quartets")
         return int(out.item())
 
+    def run_JK(self, Dc):
+        """run_J and run_K; with ``concurrent`` (and no per-launch timer or
+        progress callback) the exchange launches run on a second stream that
+        forks from and joins the current one."""
+        t = self.torch
+        if not (self.concurrent and self.timer is None and self.progress is None):
+            return self.run_J(Dc), self.run_K(Dc)
+        cur = t.cuda.current_stream(self.dev.device)
+        key = cur.cuda_stream
+        if key not in self._kstreams:
+            self._kstreams[key] = t.cuda.Stream(device=self.dev.device)
+        side = self._kstreams[key]
+        side.wait_stream(cur)
+        mJ = self.run_J(Dc)
+        with t.cuda.stream(side):
+            maxD = self.max_D(Dc)
+            mK = self.run_K(Dc, maxD=maxD)
+        cur.wait_stream(side)
+        for v in mK.values():
+            v.record_stream(cur)       # allocated on the side stream, read on this one
+        return mJ, mK
+
     def fock(self, D_sph, H=None, cJ=2.0, cK=-1.0):
         """F = H + c_J J + c_K K in the output basis for density D (device
         tensors; H optional, the core Hamiltonian is not computed here).  The
@@ -361,8 +387,7 @@ class FockEngine:
         if self.precision == "mixed":
             self.reset_counts()
         Dc = self.cart_density(D_sph)
-        mJ = self.run_J(Dc)
-        mK = self.run_K(Dc)
+        mJ, mK = self.run_JK(Dc)
         vals = t.cat([self._cat(mJ, self.j_order, {pc: self.jl[pc]["count"] for pc in self.j_order}),
                       self._cat(mK, self.k_order, {pc: self.ik[pc]["count"] for pc in self.k_order})])
         G = t.zeros(n * n, dtype=t.float64, device=self.dev.device)
@@ -399,14 +424,11 @@ class FockEngine:
         if self.precision == "mixed":
             self.reset_counts()
         Dc = self.cart_density(D_sph)
-        mJ = self.run_J(Dc)
-        Jc = self.assemble_J(mJ)
-        J = self.T @ Jc @ self.T.t()
-        K = None
-        if want_k:
-            mK = self.run_K(Dc)
-            Kc = self.assemble_K(mK)
-            K = self.T @ Kc @ self.T.t()
+        if not want_k:
+            return self.T @ self.assemble_J(self.run_J(Dc)) @ self.T.t(), None
+        mJ, mK = self.run_JK(Dc)
+        J = self.T @ self.assemble_J(mJ) @ self.T.t()
+        K = self.T @ self.assemble_K(mK) @ self.T.t()
         return J, K
 
     def build_with_report(self, D_sph):

@@ -66,6 +66,20 @@ BOYS_LDS_TABLE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "boys_
 # LDS Boys rows), chosen from per-class timings
 WPG_TABLE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "wpg.json")
 
+# gradient classes whose kernels run one pass per ket-first-index component
+# (emit_grad.grad_groups); the others run one pass.  Chosen from per-class
+# timings of both on lattice64.
+GRAD_PASSES_TABLE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "grad_passes.json")
+
+
+def load_grad_passes():
+    """{"J": {class: passes}, "K": {class: passes}}; absent classes run one pass."""
+    tab = {}
+    if os.path.exists(GRAD_PASSES_TABLE):
+        with open(GRAD_PASSES_TABLE) as f:
+            tab = json.load(f)
+    return {k: dict(tab.get(k, {})) for k in ("J", "K")}
+
 
 FAMILIES = ("J", "K", "Jb", "Kb")     # class launchers (+ _mixed), batched (+ _mixed)
 
@@ -140,10 +154,12 @@ def sources(tiles=None, layouts=None):
             out[fn] = inc + emit.emit_k(cls, kt=tiles.get(fn), layout=layouts.get(cls, "reg"),
                                         dorder=dorders.get(cls, "jl"), lds_boys=cls in blds[fam],
                                         wpg=int(wpg[fam].get(cls, 1)), **kw)
+    gp = load_grad_passes()
     for ab, cd in j_classes():
-        out["jg_%s.hip" % j_name(ab, cd)] = inc + emit_grad.emit_jgrad(j_name(ab, cd))
+        cls = j_name(ab, cd)
+        out["jg_%s.hip" % cls] = inc + emit_grad.emit_jgrad(cls, passes=gp["J"].get(cls, "one"))
     for cls in k_classes():
-        out["kg_%s.hip" % cls] = inc + emit_grad.emit_kgrad(cls)
+        out["kg_%s.hip" % cls] = inc + emit_grad.emit_kgrad(cls, passes=gp["K"].get(cls, "one"))
     for pc in PAIR_CLASSES:
         out["schwarz_%s.hip" % pc] = inc + emit.emit_schwarz(pc)
     out["jdispatch.hip"] = inc + jdispatch_text()

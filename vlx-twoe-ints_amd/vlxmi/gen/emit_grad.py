@@ -25,16 +25,14 @@ exchange energy gradients without new list data:
 Screening is the Fock kernels': a component quartet counts iff
 |Q_ij Q_kl D_kl| > thr (J, gen_coulomb.py:209-212) or |Q_ij Q_kl maxD_bd| > thr
 (K, gen_exchange.py:229-232).  The raised and lowered integrals come from the
-same concrete-component DAG as the Fock kernels (one more Boys order), in
-passes over groups of ket component pairs when the recursion does not fit the
-registers.  FP64 throughout.
+same concrete-component DAG as the Fock kernels (one more Boys order), in one
+pass or in one pass per ket-first-index component, a per-class choice
+(``grad_groups``, ``grad_passes.json``).  FP64 throughout.
 """
 
 from __future__ import annotations
 
-import re as _re
-
-from .emit import (KET_HEADER, KET_TILE, KREC0, LV, NC, REG_BUDGET, _argnames, _dag_lines, _strip_restrict,
+from .emit import (KET_HEADER, KET_TILE, KREC0, LV, NC, _argnames, _dag_lines, _strip_restrict,
                    j_signature, k_signature, ket_loops, launder, quartet_prologue)
 from .osdag import QuartetDAG, add, tri
 
@@ -75,72 +73,30 @@ def _deriv_terms(dag, pos, e, fixed, g, two_exp):
     return t
 
 
-def _grad_uses(la, lb, lc, ld, group, centres):
-    """The component quartets of one pass (ket component pairs ``group``,
-    t = ic * nd + id)."""
-    nd = (ld + 1) * (ld + 2) // 2
-    for t in group:
-        cc, dd = tri(lc)[t // nd], tri(ld)[t % nd]
-        for aa in tri(la):
-            for bb in tri(lb):
-                yield (aa, bb, cc, dd)
+def grad_groups(la, lb, lc, ld, passes="one"):
+    """The passes of a gradient kernel over the ket component pairs
+    (t = ic * nd + id): "one" -- a single pass; "component" -- one pass per
+    ket-first-index component ic (the recursion recomputed per pass, fewer
+    values live at a time).  Neither fits the registers for the p/d classes;
+    which runs faster is measured per class (``grad_passes.json``)."""
+    nc, nd = (lc + 1) * (lc + 2) // 2, (ld + 1) * (ld + 2) // 2
+    if passes == "component":
+        return [list(range(i * nd, (i + 1) * nd)) for i in range(nc)]
+    assert passes == "one", passes
+    return [list(range(nc * nd))]
 
 
-def _live_estimate_grad(la, lb, lc, ld, group, centres):
-    g = QuartetDAG(la, lb, lc, ld, rule_vrr="first", rule_hrr="first")
-    for comp in _grad_uses(la, lb, lc, ld, group, centres):
-        for pos in centres:
-            e = comp[pos]
-            for ax in range(3):
-                g.lines.append("USE %s;" % _deriv_terms(g, pos, e, comp, ax, "t"))
-    last = {}
-    for i, ln in enumerate(g.lines):
-        rhs = ln.split("=", 1)[1] if ln.startswith("const") else ln
-        for u in _re.findall(r"\bq\d+\b", rhs):
-            last[u] = i
-    live = mx = 0
-    ends = {}
-    for v, i in last.items():
-        ends.setdefault(i, []).append(v)
-    for i, ln in enumerate(g.lines):
-        m = _re.match(r"const \w+ (q\d+) = ", ln)
-        if m and m.group(1) in last:
-            live += 1
-            mx = max(mx, live)
-        live -= len(ends.get(i, []))
-    return mx
+def ket_tile(rec_doubles, extra_bytes=0):
+    """Largest ket tile (power of two, at most 64 records) whose LDS stays
+    within 32 KB, so that four single-wavefront workgroups (the most that
+    512-register kernels reach) fit a CU's 160 KB."""
+    kt = KET_TILE
+    while kt > 8 and kt * (8 * rec_doubles + extra_bytes) > 32 * 1024:
+        kt //= 2
+    return kt
 
 
-_PLAN = {}
-
-
-def plan_grad_slices(la, lb, lc, ld, kind, budget=REG_BUDGET):
-    """Fewest passes over the ket component pairs (t = ic * nd + id, in
-    contiguous equal groups) whose live recursion values plus the per-lane
-    persistent values fit ``budget`` doubles; one pair per pass when none
-    fits.  Passes that fit the registers also keep the compiler's register
-    allocation (the bulk of the compile time of these kernels) fast."""
-    key = (la, lb, lc, ld, kind, budget)
-    if key in _PLAN:
-        return _PLAN[key]
-    na, nb, nc, nd = [(l + 1) * (l + 2) // 2 for l in (la, lb, lc, ld)]
-    ncd = nc * nd
-    centres = (0, 1) if kind == "J" else (0, 2)
-    pers = 2 * na * nb + 6 + 40 + la + lb + lc + ld + 1
-    best = [[t] for t in range(ncd)]
-    for nsl in range(1, ncd + 1):
-        if ncd % nsl:
-            continue
-        size = ncd // nsl
-        groups = [list(range(k * size, (k + 1) * size)) for k in range(nsl)]
-        if max(_live_estimate_grad(la, lb, lc, ld, g_, centres) for g_ in groups) + pers <= budget:
-            best = groups
-            break
-    _PLAN[key] = best
-    return best
-
-
-def emit_jgrad(cls):
+def emit_jgrad(cls, passes="one"):
     """Source of computeCoulombGradient{CLS} and its launcher.
 
     Kernel: blockDim 64, gridDim (ceil(bra blocks / 64), ket chunks); chunk y
@@ -158,12 +114,12 @@ def emit_jgrad(cls):
     rec += rec % 2
     name = "computeCoulombGradient" + cls.upper()
     sig = jg_signature(cls)
-    groups = plan_grad_slices(la, lb, lc, ld, "J")
+    groups = grad_groups(la, lb, lc, ld, passes)
     o = ["__global__ void __launch_bounds__(64)",
          "%s(%s)" % (name, ",\n    ".join(sig)),
          "{",
          "    constexpr int NA = %d, NB = %d, NC = %d, ND = %d, NAB = %d, NCD = %d;" % (na, nb, nc, nd, nab, ncd),
-         "    constexpr int KT = %d, REC = %d;" % (KET_TILE, rec),
+         "    constexpr int KT = %d, REC = %d;" % (ket_tile(rec), rec),
          "    __shared__ double ket[KT * REC];",
          "    const uint32_t lane = threadIdx.x;",
          "    const uint32_t nbra = %s_prim_pair_count_local / NAB;" % ab,
@@ -310,7 +266,7 @@ def emit_jgrad(cls):
     return "\n".join(o) + "\n"
 
 
-def emit_kgrad(cls):
+def emit_kgrad(cls, passes="one"):
     """Source of computeExchangeGradient{CLS} and its launcher.
 
     Kernel: blockDim 64, gridDim = (i,k) blocks; lane 0 adds the wavefront's
@@ -326,8 +282,8 @@ def emit_kgrad(cls):
     rec += rec % 2
     name = "computeExchangeGradient" + cls.upper()
     sig = kg_signature(cls)
-    groups = plan_grad_slices(la, lb, lc, ld, "K")
-    kt = KET_TILE if len(groups) == 1 else 32
+    groups = grad_groups(la, lb, lc, ld, passes)
+    kt = ket_tile(rec, 4 * nd)
     o = ["__global__ void __launch_bounds__(64)",
          "%s(%s)" % (name, ",\n    ".join(sig)),
          "{",

@@ -266,12 +266,15 @@ def emit_jgrad(cls, passes="one"):
     return "\n".join(o) + "\n"
 
 
-def emit_kgrad(cls, passes="one"):
+def emit_kgrad(cls, passes="one", split=False):
     """Source of computeExchangeGradient{CLS} and its launcher.
 
     Kernel: blockDim 64, gridDim = (i,k) blocks; lane 0 adds the wavefront's
     six sums to grad[blk * 6 + 0..5] (A-centre x, y, z, then C-centre x, y, z);
-    one wavefront per block, so one writer per row."""
+    one wavefront per block, so one writer per row.  With ``split`` the
+    A-centre and C-centre parts are two kernels (``_A``, ``_C``), each with
+    one raised recursion (a+1 on the bra, or c+1 on the ket) instead of both,
+    which the launcher runs one after the other."""
     a, b, c, d = cls
     la, lb, lc, ld = LV[a], LV[b], LV[c], LV[d]
     na, nb, nc, nd = NC[a], NC[b], NC[c], NC[d]
@@ -284,172 +287,182 @@ def emit_kgrad(cls, passes="one"):
     sig = kg_signature(cls)
     groups = grad_groups(la, lb, lc, ld, passes)
     kt = ket_tile(rec, 4 * nd)
-    o = ["__global__ void __launch_bounds__(64)",
-         "%s(%s)" % (name, ",\n    ".join(sig)),
-         "{",
-         "    constexpr int NA = %d, NB = %d, NC = %d, ND = %d, NAC = %d, NAB = %d, NCD = %d;" % (na, nb, nc, nd, nac, nab, ncd),
-         "    constexpr int KT = %d, REC = %d;" % (kt, rec),
-         "    __shared__ double ket[KT * REC];",
-         "    __shared__ uint32_t ket_cg[KT * ND];",
-         "    const uint32_t lane = threadIdx.x;",
-         "    const uint32_t blk = blockIdx.x;",
-         "    if ((blk + 1) * NAC > pair_inds_count_for_K_%s) return;" % ac,
-         "    const double maxD = %s_max_D;" % bd,
-         "    if (!(maxD > 0.0)) return;",
-         "    const double* __restrict__ Dm = mat_D_full_AO;",
-         "    const uint32_t nao = naos;",
-         "    const double thr = eri_threshold;",
-         "    const double thr_eff = thr / maxD;",
-         "    const uint32_t e0 = blk * NAC;",
-         "    const uint32_t I = pair_inds_i_for_K_%s[e0] / NA;" % ac,
-         "    const uint32_t K = pair_inds_k_for_K_%s[e0] / NC;" % ac,
-         "    const double ai = %s_prim_info[I];" % a,
-         "    const double A_x = %s_prim_info[I + %s_prim_count * 2], A_y = %s_prim_info[I + %s_prim_count * 3], A_z = %s_prim_info[I + %s_prim_count * 4];" % ((a,) * 6),
-         "    const double ak = %s_prim_info[K];" % c,
-         "    const double C_x = %s_prim_info[K + %s_prim_count * 2], C_y = %s_prim_info[K + %s_prim_count * 3], C_z = %s_prim_info[K + %s_prim_count * 4];" % ((c,) * 6),
-         "    const double two_ai = 2.0 * ai, two_ak = 2.0 * ak;",
-         "    const double mIK = %s;" % ("(I == K) ? 1.0 : 2.0" if a == c else "2.0"),
-         "    double Dac[NAC];",
-         "    #pragma unroll",
-         "    for (int ca = 0; ca < NA; ++ca) {",
-         "        const uint32_t ao_a = %s_prim_aoinds[I + %s_prim_count * ca];" % (a, a),
-         "        #pragma unroll",
-         "        for (int cc = 0; cc < NC; ++cc)",
-         "            Dac[ca * NC + cc] = mIK * Dm[(size_t)ao_a * nao + %s_prim_aoinds[K + %s_prim_count * cc]];" % (c, c),
-         "    }",
-         "    uint32_t displ_i[NA], displ_k[NC];",
-         "    #pragma unroll",
-         "    for (int t = 0; t < NA; ++t) displ_i[t] = pair_displs_K_%s[I * NA + t];" % ab,
-         "    #pragma unroll",
-         "    for (int t = 0; t < NC; ++t) displ_k[t] = pair_displs_K_%s[K * NC + t];" % cd,
-         "    const uint32_t nJ = pair_counts_K_%s[I * NA] / NB;" % ab,
-         "    const uint32_t nL = pair_counts_K_%s[K * NC] / ND;" % cd,
-         "    double gacc[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};",
-         "    for (uint32_t jc = 0; jc < nJ; jc += 64) {",
-         "        const uint32_t jp = jc + lane;",
-         "        bool active = jp < nJ;",
-         "        double zeta = 1.0, sq_zeta = 1.0, inv_S1 = 1.0, P_x = 0.0, P_y = 0.0, P_z = 0.0;",
-         "        double PA_x = 0.0, PA_y = 0.0, PA_z = 0.0, AB_x = 0.0, AB_y = 0.0, AB_z = 0.0;",
-         "        double qij = 0.0;",
-         "        double tau[NAB], Sab[NAB];",
-         "        #pragma unroll",
-         "        for (int t = 0; t < NAB; ++t) { tau[t] = 0.0; Sab[t] = 0.0; }",
-         "        uint32_t cgJ[NB];",
-         "        #pragma unroll",
-         "        for (int t = 0; t < NB; ++t) cgJ[t] = 0;",
-         "        if (active) {",
-         "            const uint32_t jcomb = D_inds_K_%s[displ_i[0] + jp * NB];" % ab,
-         "            const uint32_t J = jcomb / NB;",
-         "            const double aj = %s_prim_info[J];" % b,
-         "            const double B_x = %s_prim_info[J + %s_prim_count * 2], B_y = %s_prim_info[J + %s_prim_count * 3], B_z = %s_prim_info[J + %s_prim_count * 4];" % ((b,) * 6),
-         "            zeta = ai + aj; inv_S1 = 1.0 / zeta; sq_zeta = sqrt(zeta);",
-         "            P_x = (ai * A_x + aj * B_x) * inv_S1; P_y = (ai * A_y + aj * B_y) * inv_S1; P_z = (ai * A_z + aj * B_z) * inv_S1;",
-         "            PA_x = (aj * inv_S1) * (B_x - A_x); PA_y = (aj * inv_S1) * (B_y - A_y); PA_z = (aj * inv_S1) * (B_z - A_z);",
-         "            AB_x = A_x - B_x; AB_y = A_y - B_y; AB_z = A_z - B_z;",
-         "            #pragma unroll",
-         "            for (int ca = 0; ca < NA; ++ca) {",
-         "                #pragma unroll",
-         "                for (int cb = 0; cb < NB; ++cb) {",
-         "                    const uint32_t pos = displ_i[ca] + jp * NB + cb;",
-         "                    const double q = fabs(Q_K_%s[pos]);" % ab,
-         "                    qij = fmax(qij, q);",
-         "                    tau[ca * NB + cb] = thr_eff / q;",
-         "                    Sab[ca * NB + cb] = pair_data_K_%s[pos];" % ab,
-         "                }",
-         "            }",
-         "            #pragma unroll",
-         "            for (int cb = 0; cb < NB; ++cb) {",
-         "                const uint32_t jj = D_inds_K_%s[displ_i[0] + jp * NB + cb];" % ab]
-    if b == "s":
-        o.append("                cgJ[cb] = %s_prim_aoinds[jj];" % b)
-    else:
-        o.append("                cgJ[cb] = %s_prim_aoinds[(jj / NB) + %s_prim_count * (jj %% NB)];" % (b, b))
-    o += ["            }",
-          "        }",
-          "        for (uint32_t t0 = 0; t0 < nL; t0 += KT) {",
-          "            if (!__any(active)) break;",
-          "            __syncthreads();",
-          "            {",
-          "                const uint32_t lp = t0 + lane;",
-          "                double* R = ket + lane * REC;",
-          "                if (lane < KT && lp < nL) {",
-          "                    const uint32_t lcomb = D_inds_K_%s[displ_k[0] + lp * ND];" % cd,
-          "                    const uint32_t Lp = lcomb / ND;",
-          "                    const double al = %s_prim_info[Lp];" % d,
-          "                    const double D_x = %s_prim_info[Lp + %s_prim_count * 2], D_y = %s_prim_info[Lp + %s_prim_count * 3], D_z = %s_prim_info[Lp + %s_prim_count * 4];" % ((d,) * 6),
-          "                    const double et = ak + al, iet = 1.0 / et;",
-          "                    R[1] = et; R[2] = iet; R[12] = sqrt(et);",
-          "                    R[3] = (ak * C_x + al * D_x) * iet; R[4] = (ak * C_y + al * D_y) * iet; R[5] = (ak * C_z + al * D_z) * iet;",
-          "                    R[6] = (al * iet) * (D_x - C_x); R[7] = (al * iet) * (D_y - C_y); R[8] = (al * iet) * (D_z - C_z);",
-          "                    R[9] = C_x - D_x; R[10] = C_y - D_y; R[11] = C_z - D_z;",
-          "                    double qk = 0.0;",
-          "                    #pragma unroll",
-          "                    for (int cc = 0; cc < NC; ++cc) {",
-          "                        #pragma unroll",
-          "                        for (int cdd = 0; cdd < ND; ++cdd) {",
-          "                            const uint32_t pos = displ_k[cc] + lp * ND + cdd;",
-          "                            const double q = fabs(Q_K_%s[pos]);" % cd,
-          "                            qk = fmax(qk, q);",
-          "                            R[%d + cc * ND + cdd] = pair_data_K_%s[pos];" % (KREC0, cd),
-          "                            R[%d + NCD + cc * ND + cdd] = q;" % KREC0,
-          "                        }",
-          "                    }",
-          "                    R[0] = qk;",
-          "                    #pragma unroll",
-          "                    for (int cdd = 0; cdd < ND; ++cdd) {",
-          "                        const uint32_t ll = D_inds_K_%s[displ_k[0] + lp * ND + cdd];" % cd]
-    if d == "s":
-        o.append("                        ket_cg[lane * ND + cdd] = %s_prim_aoinds[ll];" % d)
-    else:
-        o.append("                        ket_cg[lane * ND + cdd] = %s_prim_aoinds[(ll / ND) + %s_prim_count * (ll %% ND)];" % (d, d))
-    o += ["                    }",
-          "                }",
-          "            }",
-          "            __syncthreads();",
-          "            const uint32_t nt = (nL - t0 < (uint32_t)KT) ? (nL - t0) : (uint32_t)KT;"]
+    def kernel(kname, cents):
+        o = ["__global__ void __launch_bounds__(64)",
+             "%s(%s)" % (kname, ",\n    ".join(sig)),
+             "{",
+             "    constexpr int NA = %d, NB = %d, NC = %d, ND = %d, NAC = %d, NAB = %d, NCD = %d;" % (na, nb, nc, nd, nac, nab, ncd),
+             "    constexpr int KT = %d, REC = %d;" % (kt, rec),
+             "    __shared__ double ket[KT * REC];",
+             "    __shared__ uint32_t ket_cg[KT * ND];",
+             "    const uint32_t lane = threadIdx.x;",
+             "    const uint32_t blk = blockIdx.x;",
+             "    if ((blk + 1) * NAC > pair_inds_count_for_K_%s) return;" % ac,
+             "    const double maxD = %s_max_D;" % bd,
+             "    if (!(maxD > 0.0)) return;",
+             "    const double* __restrict__ Dm = mat_D_full_AO;",
+             "    const uint32_t nao = naos;",
+             "    const double thr = eri_threshold;",
+             "    const double thr_eff = thr / maxD;",
+             "    const uint32_t e0 = blk * NAC;",
+             "    const uint32_t I = pair_inds_i_for_K_%s[e0] / NA;" % ac,
+             "    const uint32_t K = pair_inds_k_for_K_%s[e0] / NC;" % ac,
+             "    const double ai = %s_prim_info[I];" % a,
+             "    const double A_x = %s_prim_info[I + %s_prim_count * 2], A_y = %s_prim_info[I + %s_prim_count * 3], A_z = %s_prim_info[I + %s_prim_count * 4];" % ((a,) * 6),
+             "    const double ak = %s_prim_info[K];" % c,
+             "    const double C_x = %s_prim_info[K + %s_prim_count * 2], C_y = %s_prim_info[K + %s_prim_count * 3], C_z = %s_prim_info[K + %s_prim_count * 4];" % ((c,) * 6),
+             "    const double two_ai = 2.0 * ai, two_ak = 2.0 * ak;",
+             "    const double mIK = %s;" % ("(I == K) ? 1.0 : 2.0" if a == c else "2.0"),
+             "    double Dac[NAC];",
+             "    #pragma unroll",
+             "    for (int ca = 0; ca < NA; ++ca) {",
+             "        const uint32_t ao_a = %s_prim_aoinds[I + %s_prim_count * ca];" % (a, a),
+             "        #pragma unroll",
+             "        for (int cc = 0; cc < NC; ++cc)",
+             "            Dac[ca * NC + cc] = mIK * Dm[(size_t)ao_a * nao + %s_prim_aoinds[K + %s_prim_count * cc]];" % (c, c),
+             "    }",
+             "    uint32_t displ_i[NA], displ_k[NC];",
+             "    #pragma unroll",
+             "    for (int t = 0; t < NA; ++t) displ_i[t] = pair_displs_K_%s[I * NA + t];" % ab,
+             "    #pragma unroll",
+             "    for (int t = 0; t < NC; ++t) displ_k[t] = pair_displs_K_%s[K * NC + t];" % cd,
+             "    const uint32_t nJ = pair_counts_K_%s[I * NA] / NB;" % ab,
+             "    const uint32_t nL = pair_counts_K_%s[K * NC] / ND;" % cd,
+             "    double gacc[%d] = {%s};" % (3 * len(cents), ", ".join(["0.0"] * (3 * len(cents)))),
+             "    for (uint32_t jc = 0; jc < nJ; jc += 64) {",
+             "        const uint32_t jp = jc + lane;",
+             "        bool active = jp < nJ;",
+             "        double zeta = 1.0, sq_zeta = 1.0, inv_S1 = 1.0, P_x = 0.0, P_y = 0.0, P_z = 0.0;",
+             "        double PA_x = 0.0, PA_y = 0.0, PA_z = 0.0, AB_x = 0.0, AB_y = 0.0, AB_z = 0.0;",
+             "        double qij = 0.0;",
+             "        double tau[NAB], Sab[NAB];",
+             "        #pragma unroll",
+             "        for (int t = 0; t < NAB; ++t) { tau[t] = 0.0; Sab[t] = 0.0; }",
+             "        uint32_t cgJ[NB];",
+             "        #pragma unroll",
+             "        for (int t = 0; t < NB; ++t) cgJ[t] = 0;",
+             "        if (active) {",
+             "            const uint32_t jcomb = D_inds_K_%s[displ_i[0] + jp * NB];" % ab,
+             "            const uint32_t J = jcomb / NB;",
+             "            const double aj = %s_prim_info[J];" % b,
+             "            const double B_x = %s_prim_info[J + %s_prim_count * 2], B_y = %s_prim_info[J + %s_prim_count * 3], B_z = %s_prim_info[J + %s_prim_count * 4];" % ((b,) * 6),
+             "            zeta = ai + aj; inv_S1 = 1.0 / zeta; sq_zeta = sqrt(zeta);",
+             "            P_x = (ai * A_x + aj * B_x) * inv_S1; P_y = (ai * A_y + aj * B_y) * inv_S1; P_z = (ai * A_z + aj * B_z) * inv_S1;",
+             "            PA_x = (aj * inv_S1) * (B_x - A_x); PA_y = (aj * inv_S1) * (B_y - A_y); PA_z = (aj * inv_S1) * (B_z - A_z);",
+             "            AB_x = A_x - B_x; AB_y = A_y - B_y; AB_z = A_z - B_z;",
+             "            #pragma unroll",
+             "            for (int ca = 0; ca < NA; ++ca) {",
+             "                #pragma unroll",
+             "                for (int cb = 0; cb < NB; ++cb) {",
+             "                    const uint32_t pos = displ_i[ca] + jp * NB + cb;",
+             "                    const double q = fabs(Q_K_%s[pos]);" % ab,
+             "                    qij = fmax(qij, q);",
+             "                    tau[ca * NB + cb] = thr_eff / q;",
+             "                    Sab[ca * NB + cb] = pair_data_K_%s[pos];" % ab,
+             "                }",
+             "            }",
+             "            #pragma unroll",
+             "            for (int cb = 0; cb < NB; ++cb) {",
+             "                const uint32_t jj = D_inds_K_%s[displ_i[0] + jp * NB + cb];" % ab]
+        if b == "s":
+            o.append("                cgJ[cb] = %s_prim_aoinds[jj];" % b)
+        else:
+            o.append("                cgJ[cb] = %s_prim_aoinds[(jj / NB) + %s_prim_count * (jj %% NB)];" % (b, b))
+        o += ["            }",
+              "        }",
+              "        for (uint32_t t0 = 0; t0 < nL; t0 += KT) {",
+              "            if (!__any(active)) break;",
+              "            __syncthreads();",
+              "            {",
+              "                const uint32_t lp = t0 + lane;",
+              "                double* R = ket + lane * REC;",
+              "                if (lane < KT && lp < nL) {",
+              "                    const uint32_t lcomb = D_inds_K_%s[displ_k[0] + lp * ND];" % cd,
+              "                    const uint32_t Lp = lcomb / ND;",
+              "                    const double al = %s_prim_info[Lp];" % d,
+              "                    const double D_x = %s_prim_info[Lp + %s_prim_count * 2], D_y = %s_prim_info[Lp + %s_prim_count * 3], D_z = %s_prim_info[Lp + %s_prim_count * 4];" % ((d,) * 6),
+              "                    const double et = ak + al, iet = 1.0 / et;",
+              "                    R[1] = et; R[2] = iet; R[12] = sqrt(et);",
+              "                    R[3] = (ak * C_x + al * D_x) * iet; R[4] = (ak * C_y + al * D_y) * iet; R[5] = (ak * C_z + al * D_z) * iet;",
+              "                    R[6] = (al * iet) * (D_x - C_x); R[7] = (al * iet) * (D_y - C_y); R[8] = (al * iet) * (D_z - C_z);",
+              "                    R[9] = C_x - D_x; R[10] = C_y - D_y; R[11] = C_z - D_z;",
+              "                    double qk = 0.0;",
+              "                    #pragma unroll",
+              "                    for (int cc = 0; cc < NC; ++cc) {",
+              "                        #pragma unroll",
+              "                        for (int cdd = 0; cdd < ND; ++cdd) {",
+              "                            const uint32_t pos = displ_k[cc] + lp * ND + cdd;",
+              "                            const double q = fabs(Q_K_%s[pos]);" % cd,
+              "                            qk = fmax(qk, q);",
+              "                            R[%d + cc * ND + cdd] = pair_data_K_%s[pos];" % (KREC0, cd),
+              "                            R[%d + NCD + cc * ND + cdd] = q;" % KREC0,
+              "                        }",
+              "                    }",
+              "                    R[0] = qk;",
+              "                    #pragma unroll",
+              "                    for (int cdd = 0; cdd < ND; ++cdd) {",
+              "                        const uint32_t ll = D_inds_K_%s[displ_k[0] + lp * ND + cdd];" % cd]
+        if d == "s":
+            o.append("                        ket_cg[lane * ND + cdd] = %s_prim_aoinds[ll];" % d)
+        else:
+            o.append("                        ket_cg[lane * ND + cdd] = %s_prim_aoinds[(ll / ND) + %s_prim_count * (ll %% ND)];" % (d, d))
+        o += ["                    }",
+              "                }",
+              "            }",
+              "            __syncthreads();",
+              "            const uint32_t nt = (nL - t0 < (uint32_t)KT) ? (nL - t0) : (uint32_t)KT;"]
 
-    def body(tyq, ind):
-        lines = quartet_prologue(L, omega=True, ind=ind)
-        for ibb in range(nb):
-            for idd in range(nd):
-                lines.append(ind + "const double Dj%d_%d = Dm[(size_t)cgJ[%d] * nao + ket_cg[s * ND + %d]];" % (ibb, idd, ibb, idd))
-        for isl, group in enumerate(groups):
-            if isl:
-                lines += [ind + x for x in launder(L)]
-            dag = QuartetDAG(la, lb, lc, ld, prefix="q%d_" % isl, rule_vrr="first", rule_hrr="first")
-            dig = []
-            for tcd in group:
-                ic, idd = divmod(tcd, nd)
-                cc, dd = tri(lc)[ic], tri(ld)[idd]
-                dig.append("const double Sc%d = R[%d], Qk%d = R[%d];" % (tcd, KREC0 + tcd, tcd, KREC0 + ncd + tcd))
-                for ia, aa in enumerate(tri(la)):
-                    for ib, bbx in enumerate(tri(lb)):
-                        tab = ia * nb + ib
-                        tac = ia * nc + ic
-                        comp = (aa, bbx, cc, dd)
-                        cf = "cf%d_%d" % (tab, tcd)
-                        dag.lines.append("const double %s = (Qk%d > tau[%d]) ? (Dac[%d] * Sab[%d] * Sc%d * Dj%d_%d) : 0.0;"
-                                         % (cf, tcd, tab, tac, tab, tcd, ib, idd))
-                        for g in range(3):
-                            dag.lines.append("gacc[%d] += %s * (%s);" % (g, cf, _deriv_terms(dag, 0, aa, comp, g, "two_ai")))
-                            dag.lines.append("gacc[%d] += %s * (%s);" % (3 + g, cf, _deriv_terms(dag, 2, cc, comp, g, "two_ak")))
-            lines += [ind + x for x in dig]
-            lines += [ind + x for x in _dag_lines(dag)]
-        return lines
+        def body(tyq, ind):
+            lines = quartet_prologue(L, omega=True, ind=ind)
+            for ibb in range(nb):
+                for idd in range(nd):
+                    lines.append(ind + "const double Dj%d_%d = Dm[(size_t)cgJ[%d] * nao + ket_cg[s * ND + %d]];" % (ibb, idd, ibb, idd))
+            for isl, group in enumerate(groups):
+                if isl:
+                    lines += [ind + x for x in launder(L)]
+                dag = QuartetDAG(la, lb, lc, ld, prefix="q%d_" % isl, rule_vrr="first", rule_hrr="first")
+                dig = []
+                for tcd in group:
+                    ic, idd = divmod(tcd, nd)
+                    cc, dd = tri(lc)[ic], tri(ld)[idd]
+                    dig.append("const double Sc%d = R[%d], Qk%d = R[%d];" % (tcd, KREC0 + tcd, tcd, KREC0 + ncd + tcd))
+                    for ia, aa in enumerate(tri(la)):
+                        for ib, bbx in enumerate(tri(lb)):
+                            tab = ia * nb + ib
+                            tac = ia * nc + ic
+                            comp = (aa, bbx, cc, dd)
+                            cf = "cf%d_%d" % (tab, tcd)
+                            dag.lines.append("const double %s = (Qk%d > tau[%d]) ? (Dac[%d] * Sab[%d] * Sc%d * Dj%d_%d) : 0.0;"
+                                             % (cf, tcd, tab, tac, tab, tcd, ib, idd))
+                            for g in range(3):
+                                for ic_, (pos, two, _) in enumerate(cents):
+                                    dag.lines.append("gacc[%d] += %s * (%s);"
+                                                     % (3 * ic_ + g, cf, _deriv_terms(dag, pos, comp[pos], comp, g, two)))
+                lines += [ind + x for x in dig]
+                lines += [ind + x for x in _dag_lines(dag)]
+            return lines
 
-    o += ket_loops("            ", "active", "qij * R[0] * maxD * (1.0 + 1e-13) <= thr", "qij * R[0] * maxD",
-                   KET_HEADER, body, False, "lo_mode", count=False)
-    o += ["        }",
-          "    }",
-          "    #pragma unroll",
-          "    for (int g = 0; g < 6; ++g) {",
-          "        double v = gacc[g];",
-          "        #pragma unroll",
-          "        for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);",
-          "        if (lane == 0) grad_%s[(size_t)blk * 6 + g] += v;" % ac,
-          "    }",
-          "}",
-          ""]
+        o += ket_loops("            ", "active", "qij * R[0] * maxD * (1.0 + 1e-13) <= thr", "qij * R[0] * maxD",
+                       KET_HEADER, body, False, "lo_mode", count=False)
+        o += ["        }",
+              "    }",
+              "    #pragma unroll",
+              "    for (int g = 0; g < %d; ++g) {" % (3 * len(cents)),
+              "        double v = gacc[g];",
+              "        #pragma unroll",
+              "        for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);",
+              "        if (lane == 0) grad_%s[(size_t)blk * 6 + %d + g] += v;" % (ac, cents[0][2]),
+              "    }",
+              "}",
+              ""]
+        return o
+
+    # (position, 2 x exponent, first output column) of the centres a kernel covers
+    both = [(0, "two_ai", 0), (2, "two_ak", 3)]
+    parts = [(name, both)] if not split else [(name + "_A", both[:1]), (name + "_C", both[1:])]
+    o = []
+    for kname, cents in parts:
+        o += kernel(kname, cents)
     lsig = _strip_restrict(sig)
     args = _argnames(sig)
     o += ['extern "C" int vlxmi_launch_%s(%s,\n    void* stream)' % (name, ",\n    ".join(lsig)),
@@ -467,8 +480,13 @@ def emit_kgrad(cls, passes="one"):
         o.append("        if (rc == 0) rc = vlxmi_validate_k_csr(%d, %d, %s_prim_count, pair_displs_K_%s, pair_counts_K_%s, Q_K_%s, D_inds_K_%s, stream);"
                  % (lc, ld, c, cd, cd, cd, cd))
     o += ["        if (rc != 0) return rc;",
-          "    }",
-          "    hipLaunchKernelGGL(%s, dim3(nblk), dim3(64), 0, (hipStream_t)stream, %s);" % (name, ", ".join(args)),
-          "    return (int)hipGetLastError();",
+          "    }"]
+    for kn, _ in parts:
+        o += ["    hipLaunchKernelGGL(%s, dim3(nblk), dim3(64), 0, (hipStream_t)stream, %s);" % (kn, ", ".join(args)),
+              "    {",
+              "        const int rc = (int)hipGetLastError();",
+              "        if (rc != 0) return rc;",
+              "    }"]
+    o += ["    return VLXMI_OK;",
           "}"]
     return "\n".join(o) + "\n"

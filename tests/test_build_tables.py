@@ -155,3 +155,51 @@ def test_mixed_fp32_passes():
     for cls in ("sddd", "pppd"):
         la, lb, lc, ld = [emit.LV[x] for x in cls]
         assert len(emit.plan_slices_f32(la, lb, lc, ld, "J")) < len(emit.plan_slices(la, lb, lc, ld, "J"))
+
+
+def test_gradient_layout_table():
+    """Every gradient class named in grad_passes.json has a known layout, and
+    the generated sources follow it: Coulomb classes listed run one pass per
+    ket-first-index component; exchange classes split by centre have an _A
+    and a _C kernel, launched one after the other by the class launcher."""
+    from vlxmi.gen import emit_grad
+    gp = B.load_grad_passes()
+    assert set(gp["J"].values()) <= {"component"} and gp["J"]
+    assert set(gp["K"].values()) <= {"component", "split_one", "split_component"} and gp["K"]
+    assert set(gp["J"]) <= {ab + cd for ab, cd in j_classes()} and set(gp["K"]) <= set(k_classes())
+    srcs = B.sources()
+    for cls, v in gp["K"].items():
+        passes, split = B.grad_k_layout(gp, cls)
+        s = srcs["kg_%s.hip" % cls]
+        name = "computeExchangeGradient" + cls.upper()
+        if split:
+            assert (name + "_A(") in s and (name + "_C(") in s
+            assert s.count("hipLaunchKernelGGL(%s_" % name) == 2
+        nc, nd = [(emit.LV[x] + 1) * (emit.LV[x] + 2) // 2 for x in cls[2:]]
+        assert len(emit_grad.grad_groups(*[emit.LV[x] for x in cls], passes)) == (nc if passes == "component" else 1)
+    for cls in gp["J"]:
+        assert "computeCoulombGradient" + cls.upper() in srcs["jg_%s.hip" % cls]
+
+
+# Scratch per lane a kernel may use: the runtime backs it for every wave the
+# device can hold, and a 34 KB-per-lane kernel failed to dispatch
+# (out of resources) in a process with several queues.
+SCRATCH_LIMIT = 16 * 1024
+
+
+def test_kernel_scratch_within_limit():
+    """Every compiled J/K and gradient kernel needs at most SCRATCH_LIMIT bytes
+    of scratch per lane (register spills)."""
+    import glob
+    objs = glob.glob(os.path.join(B.OBJ, "*.hip.*.o"))
+    if not objs:
+        pytest.skip("library not built")
+    worst = {}
+    for o in objs:
+        if not os.path.basename(o).startswith(("j_", "k_", "jm_", "km_", "jb", "kb", "jg_", "kg_")):
+            continue
+        for name, scr in B.kernel_scratch(o).items():
+            worst[name] = scr
+    assert worst
+    over = {n: s for n, s in worst.items() if s > SCRATCH_LIMIT}
+    assert not over, over

@@ -72,6 +72,12 @@ WPG_TABLE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "wpg.json")
 GRAD_PASSES_TABLE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "grad_passes.json")
 
 
+def grad_k_layout(gp, cls):
+    """(passes, split) of the exchange-gradient kernel of ``cls``."""
+    v = gp["K"].get(cls, "one")
+    return (v[len("split_"):], True) if v.startswith("split_") else (v, False)
+
+
 def load_grad_passes():
     """{"J": {class: passes}, "K": {class: passes}}; absent classes run one pass."""
     tab = {}
@@ -159,7 +165,8 @@ def sources(tiles=None, layouts=None):
         cls = j_name(ab, cd)
         out["jg_%s.hip" % cls] = inc + emit_grad.emit_jgrad(cls, passes=gp["J"].get(cls, "one"))
     for cls in k_classes():
-        out["kg_%s.hip" % cls] = inc + emit_grad.emit_kgrad(cls, passes=gp["K"].get(cls, "one"))
+        passes, split = grad_k_layout(gp, cls)
+        out["kg_%s.hip" % cls] = inc + emit_grad.emit_kgrad(cls, passes=passes, split=split)
     for pc in PAIR_CLASSES:
         out["schwarz_%s.hip" % pc] = inc + emit.emit_schwarz(pc)
     out["jdispatch.hip"] = inc + jdispatch_text()
@@ -245,12 +252,19 @@ def device_header_text():
           " *   computeCoulombGradient{AB}{CD}: blockDim 64, gridDim (ceil(bra blocks / 64), ket chunks);",
           " *     chunk y stores the six sums of bra block bb at grad[(y * nbra + bb) * 6 + 0..5];",
           " *   computeExchangeGradient{A}{B}{C}{D}: blockDim 64, gridDim = (i,k) blocks; adds six sums",
-          " *     per block to grad[blk * 6 + 0..5]. */"]
+          " *     per block to grad[blk * 6 + 0..5]; for the classes split by centre, the _A kernel adds",
+          " *     columns 0..2 and the _C kernel columns 3..5 (the launcher runs both). */"]
     for ab, cd in j_classes():
         cls = j_name(ab, cd)
         o.append("__global__ void computeCoulombGradient%s(%s);" % (cls.upper(), ", ".join(emit._strip_restrict(emit_grad.jg_signature(cls)))))
+    gp = load_grad_passes()
     for cls in k_classes():
-        o.append("__global__ void computeExchangeGradient%s(%s);" % (cls.upper(), ", ".join(emit._strip_restrict(emit_grad.kg_signature(cls)))))
+        sig = ", ".join(emit._strip_restrict(emit_grad.kg_signature(cls)))
+        if grad_k_layout(gp, cls)[1]:
+            o.append("__global__ void computeExchangeGradient%s_A(%s);" % (cls.upper(), sig))
+            o.append("__global__ void computeExchangeGradient%s_C(%s);" % (cls.upper(), sig))
+        else:
+            o.append("__global__ void computeExchangeGradient%s(%s);" % (cls.upper(), sig))
     o += ["", "#endif", ""]
     return "\n".join(o)
 
@@ -437,6 +451,23 @@ def kernel_resources(obj_path):
         regs = int(re.search(r"\.vgpr_count:\s+(\d+)", blk).group(1))
         lds = int(re.search(r"\.group_segment_fixed_size:\s+(\d+)", blk).group(1))
         out[name] = (regs, lds)
+    return out
+
+
+def kernel_scratch(obj_path):
+    """{kernel name: scratch bytes per lane (private segment)} of the gfx950
+    code object inside a host object compiled by this module."""
+    with tempfile.TemporaryDirectory() as td:
+        fb, dv = os.path.join(td, "fb.bin"), os.path.join(td, "dev.o")
+        subprocess.run([LLVM + "/llvm-objcopy", "--dump-section", ".hip_fatbin=" + fb, obj_path], check=True)
+        subprocess.run([LLVM + "/clang-offload-bundler", "--unbundle", "--type=o", "--input=" + fb,
+                        "--output=" + dv, "--targets=hipv4-amdgcn-amd-amdhsa--" + ARCH], check=True)
+        txt = subprocess.run([LLVM + "/llvm-readelf", "--notes", dv], capture_output=True, text=True,
+                             check=True).stdout
+    out = {}
+    for blk in txt.split("- .agpr_count")[1:]:
+        name = re.search(r"\.name:\s+(\S+)", blk).group(1)
+        out[name] = int(re.search(r"\.private_segment_fixed_size:\s+(\d+)", blk).group(1))
     return out
 
 

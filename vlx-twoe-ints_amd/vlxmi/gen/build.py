@@ -66,6 +66,19 @@ BOYS_LDS_TABLE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "boys_
 # LDS Boys rows), chosen from per-class timings
 WPG_TABLE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "wpg.json")
 
+# heavy exchange classes whose plain FP64 kernel runs one pass per ket
+# component pair (emit.emit_k kpairs), chosen from lattice64 per-class timings
+K_PASSES_TABLE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "k_passes.json")
+
+
+def load_k_passes():
+    """Classes whose plain FP64 exchange kernel runs pair passes."""
+    if not os.path.exists(K_PASSES_TABLE):
+        return set()
+    with open(K_PASSES_TABLE) as f:
+        return {k for k, v in json.load(f).items() if not k.startswith("_") and v == "pair"}
+
+
 # gradient classes whose kernels run one pass per ket-first-index component
 # (emit_grad.grad_groups); the others run one pass.  Chosen from per-class
 # timings of both on lattice64.
@@ -152,8 +165,9 @@ def sources(tiles=None, layouts=None):
             fam = "Jb" if kw.get("batched") else "J"
             out[fn] = inc + emit.emit_j(cls, kt=tiles.get(fn), layout=jlayouts.get(cls, "reg"),
                                         lds_boys=cls in blds[fam], wpg=int(wpg[fam].get(cls, 1)), **kw)
+    kpairs = load_k_passes()
     for cls in k_classes():
-        for pre, kw in (("k", {}), ("km", dict(mixed=True)), ("kb", dict(batched=True)),
+        for pre, kw in (("k", dict(kpairs=cls in kpairs)), ("km", dict(mixed=True)), ("kb", dict(batched=True)),
                         ("kbm", dict(batched=True, mixed=True))):
             fn = "%s_%s.hip" % (pre, cls)
             fam = "Kb" if kw.get("batched") else "K"

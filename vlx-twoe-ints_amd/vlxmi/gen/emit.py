@@ -403,7 +403,7 @@ def ket_record_bytes(kind, cls):
     return rec * 8 + (NC[d] * 4 if kind == "K" else 0)
 
 
-def emit_j(cls, mixed=False, batched=False, kt=None, layout="reg", lds_boys=False, wpg=1):
+def emit_j(cls, mixed=False, batched=False, kt=None, layout="reg", lds_boys=False, wpg=1, jpairs=False):
     """Source of computeCoulombFock{CLS} (or its _mixed / _batched variants) and its
     C-ABI launcher.  The batched variant is persistent: workgroup w processes
     work items w, w + gridDim.x, ...; item t = (bra block begin, bra block end
@@ -473,6 +473,10 @@ def emit_j(cls, mixed=False, batched=False, kt=None, layout="reg", lds_boys=Fals
     o.append("    double PA_x = 0.0, PA_y = 0.0, PA_z = 0.0, AB_x = 0.0, AB_y = 0.0, AB_z = 0.0;")
     o.append("    double qmax = 0.0;")
     jsl = plan_slices(la, lb, lc, ld, "J")
+    if len(jsl) > 1 and jpairs:
+        # one pass per ket component pair (ic, id), recursion recomputed per pair
+        assert not mixed, "pair passes are for the FP64 kernels"
+        jsl = [[(ic, idd)] for ic in range(nc) for idd in range(nd)]
     # Classes split into passes have two layouts, chosen per class by
     # measurement: "reg" keeps the NAB accumulators and thresholds in registers;
     # "lds" keeps them in LDS ([component][lane], lane-private slots), the
@@ -560,9 +564,11 @@ def emit_j(cls, mixed=False, batched=False, kt=None, layout="reg", lds_boys=Fals
                 lines += [ind + x for x in launder(L)]
             dag = QuartetDAG(la, lb, lc, ld, ty=tyq, rule_vrr="first", rule_hrr="first", prefix="q%d_" % isl)
             dig = []
-            for ic in group:
+            for e in group:
+                ic, idds = (e[0], [e[1]]) if isinstance(e, tuple) else (e, list(range(nd)))
                 cc = tri(lc)[ic]
-                for idd, dd in enumerate(tri(ld)):
+                for idd in idds:
+                    dd = tri(ld)[idd]
                     tcd = ic * nd + idd
                     dig.append("const double Dp%d = R[%d], Qd%d = R[%d];" % (tcd, KREC0 + tcd, tcd, KREC0 + ncd + tcd))
                     for ia, aa in enumerate(tri(la)):
@@ -692,7 +698,7 @@ def emit_j(cls, mixed=False, batched=False, kt=None, layout="reg", lds_boys=Fals
 # K kernel
 # --------------------------------------------------------------------------
 
-def emit_k(cls, mixed=False, batched=False, kt=None, layout="reg", dorder="jl", lds_boys=False, wpg=1):
+def emit_k(cls, mixed=False, batched=False, kt=None, layout="reg", dorder="jl", lds_boys=False, wpg=1, kpairs=False):
     """Source of computeExchangeFock{CLS} (or its _mixed / _batched variants) and
     its launcher.  The batched variant is persistent over a list of (i,k) blocks
     of lists that concatenate many molecules; per block, blk_mol gives the
@@ -716,6 +722,11 @@ def emit_k(cls, mixed=False, batched=False, kt=None, layout="reg", dorder="jl", 
     rec += rec % 2
     kslices = plan_slices(la, lb, lc, ld, "Kheavy")
     heavy = len(kslices) > 1
+    if heavy and kpairs:
+        # one pass per ket component pair (ic, id): the recursion is recomputed
+        # per pair, and only one pair's intermediates are live at a time
+        assert not mixed, "pair passes are for the FP64 kernels"
+        kslices = [[(ic, idd)] for ic in range(nc) for idd in range(nd)]
     name = "computeExchangeFock" + cls.upper() + ("_batched" if batched else "") + ("_mixed" if mixed else "")
     bsig = ["const uint32_t* __restrict__ blk_list", "const uint32_t nlist", "const uint32_t* __restrict__ blk_mol",
             "const double* __restrict__ mol_maxD", "const unsigned long long* __restrict__ mol_D_off",
@@ -907,6 +918,14 @@ def emit_k(cls, mixed=False, batched=False, kt=None, layout="reg", dorder="jl", 
             return "Dm[(size_t)%s * nao + ket_cg[s * ND + %d]]" % (jrow, idd)
         return "Dm[(size_t)ket_cg[s * ND + %d] * nao + %s]" % (idd, jrow)
 
+    def comps(group):
+        """(ic, [id...]) of a pass: an element ic stands for all id."""
+        out = {}
+        for e in group:
+            ic, ids = (e[0], [e[1]]) if isinstance(e, tuple) else (e, list(range(nd)))
+            out.setdefault(ic, []).extend(ids)
+        return list(out.items())
+
     def kbody(tyq, ind, groups):
         # the density gathers follow the Boys evaluation: vector loads complete
         # in order, so gathers issued before the Boys table loads only lengthen
@@ -925,21 +944,21 @@ def emit_k(cls, mixed=False, batched=False, kt=None, layout="reg", dorder="jl", 
         # (an opaque row index keeps the loads from being hoisted)
         bouter = lds_layout
         ACC = (lambda t: "pacc%d" % t) if lacc else (lambda t: "acc[%d]" % t)
+        used = sorted({(ic_, d_) for g_ in groups for ic_, ds in comps(g_) for d_ in ds})
         if not bouter:
             for ibb in range(nb):
-                for idd in range(nd):
+                for idd in sorted({d_ for _, d_ in used}):
                     lines.append(ind + "const double Dj%d_%d = %s;"
                                  % (ibb, idd, dread(("rowJ%d" % ibb) if heavy else ("cgJ[%d]" % ibb), idd)))
         # digestion: acc[a,c] += Sab * sum_{b,d} mask * X * Sc * D  (FP64 accumulation)
-        for icc in sorted(set(c_ for g_ in groups for c_ in g_)):
-            for idd in range(nd):
-                tcd = icc * nd + idd
-                lines.append(ind + "const double Sc%d = R[%d], Qk%d = R[%d];" % (tcd, KREC0 + tcd, tcd, KREC0 + ncd + tcd))
+        for icc, idd in used:
+            tcd = icc * nd + idd
+            lines.append(ind + "const double Sc%d = R[%d], Qk%d = R[%d];" % (tcd, KREC0 + tcd, tcd, KREC0 + ncd + tcd))
         for isl, group in enumerate(groups):
             if isl:
                 lines += [ind + x for x in launder(L)]
             dag = QuartetDAG(la, lb, lc, ld, ty=tyq, rule_vrr="first", rule_hrr="first", prefix="q%d_" % isl)
-            for ic in group:
+            for ic, idds in comps(group):
                 cc = tri(lc)[ic]
                 order = ([(ia, ib) for ib in range(nb) for ia in range(na)] if bouter
                          else [(ia, ib) for ia in range(na) for ib in range(nb)])
@@ -952,11 +971,12 @@ def emit_k(cls, mixed=False, batched=False, kt=None, layout="reg", dorder="jl", 
                         # this b's density row, loaded just before its block
                         dag.lines.append("uint32_t rowB%d_%d_c%d = cgJ[%d];" % (isl, ib, ic, ib))
                         dag.lines.append("asm volatile(\"\" : \"+v\"(rowB%d_%d_c%d));" % (isl, ib, ic))
-                        for idd in range(nd):
+                        for idd in idds:
                             dag.lines.append("const double %s = %s;"
                                              % (dname(ib, idd), dread("rowB%d_%d_c%d" % (isl, ib, ic), idd)))
                     terms = []
-                    for idd, dd in enumerate(tri(ld)):
+                    for idd in idds:
+                        dd = tri(ld)[idd]
                         tcd = ic * nd + idd
                         x = dag.X(aa, bbx, cc, dd)
                         xv = x if tyq == "double" else "double(%s)" % x
@@ -970,7 +990,7 @@ def emit_k(cls, mixed=False, batched=False, kt=None, layout="reg", dorder="jl", 
     def pacc_io(group, store):
         out = []
         if lacc:
-            for ic in group:
+            for ic, _ in comps(group):
                 for ia in range(na):
                     t = ia * nc + ic
                     if store:

@@ -71,12 +71,24 @@ WPG_TABLE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "wpg.json")
 K_PASSES_TABLE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "k_passes.json")
 
 
+J_PASSES_TABLE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "j_passes.json")
+
+
+def _pair_classes(path):
+    if not os.path.exists(path):
+        return set()
+    with open(path) as f:
+        return {k for k, v in json.load(f).items() if not k.startswith("_") and v == "pair"}
+
+
 def load_k_passes():
     """Classes whose plain FP64 exchange kernel runs pair passes."""
-    if not os.path.exists(K_PASSES_TABLE):
-        return set()
-    with open(K_PASSES_TABLE) as f:
-        return {k for k, v in json.load(f).items() if not k.startswith("_") and v == "pair"}
+    return _pair_classes(K_PASSES_TABLE)
+
+
+def load_j_passes():
+    """Split Coulomb classes whose plain FP64 kernel runs pair passes."""
+    return _pair_classes(J_PASSES_TABLE)
 
 
 # gradient classes whose kernels run one pass per ket-first-index component
@@ -157,9 +169,10 @@ def sources(tiles=None, layouts=None):
     wpg = load_wpg()
     out = {}
     inc = '#include "vlxmi_device.hpp"\n\n'
+    jpairs = load_j_passes()
     for ab, cd in j_classes():
         cls = j_name(ab, cd)
-        for pre, kw in (("j", {}), ("jm", dict(mixed=True)), ("jb", dict(batched=True)),
+        for pre, kw in (("j", dict(jpairs=cls in jpairs)), ("jm", dict(mixed=True)), ("jb", dict(batched=True)),
                         ("jbm", dict(batched=True, mixed=True))):
             fn = "%s_%s.hip" % (pre, cls)
             fam = "Jb" if kw.get("batched") else "J"

@@ -164,7 +164,7 @@ def test_gradient_layout_table():
     and a _C kernel, launched one after the other by the class launcher."""
     from vlxmi.gen import emit_grad
     gp = B.load_grad_passes()
-    assert set(gp["J"].values()) <= {"component"} and gp["J"]
+    assert set(gp["J"].values()) <= {"component", "pair"} and gp["J"]
     assert set(gp["K"].values()) <= {"component", "split_one", "split_component"} and gp["K"]
     assert set(gp["J"]) <= {ab + cd for ab, cd in j_classes()} and set(gp["K"]) <= set(k_classes())
     srcs = B.sources()
@@ -177,8 +177,10 @@ def test_gradient_layout_table():
             assert s.count("hipLaunchKernelGGL(%s_" % name) == 2
         nc, nd = [(emit.LV[x] + 1) * (emit.LV[x] + 2) // 2 for x in cls[2:]]
         assert len(emit_grad.grad_groups(*[emit.LV[x] for x in cls], passes)) == (nc if passes == "component" else 1)
-    for cls in gp["J"]:
+    for cls, v in gp["J"].items():
         assert "computeCoulombGradient" + cls.upper() in srcs["jg_%s.hip" % cls]
+        nc, nd = [(emit.LV[x] + 1) * (emit.LV[x] + 2) // 2 for x in cls[2:]]
+        assert len(emit_grad.grad_groups(*[emit.LV[x] for x in cls], v)) == (nc if v == "component" else nc * nd)
 
 
 # Scratch per lane a kernel may use: the runtime backs it for every wave the

@@ -77,11 +77,13 @@ def grad_groups(la, lb, lc, ld, passes="one"):
     """The passes of a gradient kernel over the ket component pairs
     (t = ic * nd + id): "one" -- a single pass; "component" -- one pass per
     ket-first-index component ic (the recursion recomputed per pass, fewer
-    values live at a time).  Neither fits the registers for the p/d classes;
+    values live at a time); "pair" -- one pass per pair t.  Neither fits the registers for the p/d classes;
     which runs faster is measured per class (``grad_passes.json``)."""
     nc, nd = (lc + 1) * (lc + 2) // 2, (ld + 1) * (ld + 2) // 2
     if passes == "component":
         return [list(range(i * nd, (i + 1) * nd)) for i in range(nc)]
+    if passes == "pair":
+        return [[t] for t in range(nc * nd)]
     assert passes == "one", passes
     return [list(range(nc * nd))]
 

@@ -80,3 +80,40 @@ def test_signatures_follow_reference_lists():
     # ab == cd: one set only
     names = [s.split()[-1] for s in emit.k_signature("spsp")]
     assert "Q_K_sp" in names and names.count("Q_K_sp") == 1
+
+
+def _digest_terms(src):
+    """(accumulator, factor, mask, density/record factors) of every digestion
+    term of a generated Fock kernel, without the recursion value names."""
+    import re
+    out = []
+    for line in src.split("\n"):
+        m = re.match(r"\s*((?:acc\[\d+\])|(?:pacc\d+)) \+= (.+?) \* \((.*)\);\s*$", line)
+        if m and "Qk" in line:
+            acc, sab = m.group(1).replace("pacc", "acc"), m.group(2)
+            for t in re.findall(r"\(\(Qk(\d+) > (.+?)\) \? \(\S+ \* Sc(\d+) \* (Dj[\d_]+?)(?:_c\d+)?\)", m.group(3)):
+                out.append((acc, sab) + t)
+            continue
+        m = re.search(r"(acc\[\d+\]) \+= \(Qd(\d+) > tau\[(\d+)\]\) \? \(\S+ \* Dp(\d+)\)", line)
+        if m:
+            out.append(m.groups())
+        m = re.search(r"if \(Qd(\d+) > ltau\[(\d+) \* TILE_SIZE_J \+ lid\]\) __hip_atomic_fetch_add\("
+                      r"&lacc\[(\d+) \* TILE_SIZE_J \+ lid\], \S+ \* Dp(\d+)", line)
+        if m:
+            out.append(m.groups())
+    return sorted(out)
+
+
+@pytest.mark.parametrize("cls,layout", [("pddd", "reg"), ("pddd", "lds"), ("dddd", "reg"), ("pdpd", "lds")])
+def test_pair_passes_digest_the_same_terms(cls, layout):
+    """One pass per ket component pair changes only where the recursion values
+    come from: every kernel digests the same masked terms into the same
+    accumulators as with one pass per ket-first-index component."""
+    a, b = emit.emit_k(cls, layout=layout), emit.emit_k(cls, layout=layout, kpairs=True)
+    ta, tb = _digest_terms(a), _digest_terms(b)
+    assert ta and ta == tb
+    nc, nd = [(emit.LV[x] + 1) * (emit.LV[x] + 2) // 2 for x in cls[2:]]
+    assert b.count("for (; s < nt; ++s)") == nc * nd
+    a, b = emit.emit_j(cls, layout=layout), emit.emit_j(cls, layout=layout, jpairs=True)
+    ta, tb = _digest_terms(a), _digest_terms(b)
+    assert ta and ta == tb

@@ -160,12 +160,14 @@ def test_mixed_fp32_passes():
 def test_gradient_layout_table():
     """Every gradient class named in grad_passes.json has a known layout, and
     the generated sources follow it: Coulomb classes listed run one pass per
-    ket-first-index component; exchange classes split by centre have an _A
-    and a _C kernel, launched one after the other by the class launcher."""
+    ket-first-index component or per ket component pair; exchange classes
+    split by centre have an _A and a _C kernel, launched one after the other
+    by the class launcher, each with one pass, one per ket-first-index
+    component or one per ket component pair."""
     from vlxmi.gen import emit_grad
     gp = B.load_grad_passes()
     assert set(gp["J"].values()) <= {"component", "pair"} and gp["J"]
-    assert set(gp["K"].values()) <= {"component", "split_one", "split_component"} and gp["K"]
+    assert set(gp["K"].values()) <= {"component", "split_one", "split_component", "split_pair"} and gp["K"]
     assert set(gp["J"]) <= {ab + cd for ab, cd in j_classes()} and set(gp["K"]) <= set(k_classes())
     srcs = B.sources()
     for cls, v in gp["K"].items():
@@ -176,7 +178,8 @@ def test_gradient_layout_table():
             assert (name + "_A(") in s and (name + "_C(") in s
             assert s.count("hipLaunchKernelGGL(%s_" % name) == 2
         nc, nd = [(emit.LV[x] + 1) * (emit.LV[x] + 2) // 2 for x in cls[2:]]
-        assert len(emit_grad.grad_groups(*[emit.LV[x] for x in cls], passes)) == (nc if passes == "component" else 1)
+        assert len(emit_grad.grad_groups(*[emit.LV[x] for x in cls], passes)) == \
+            {"component": nc, "pair": nc * nd, "one": 1}[passes]
     for cls, v in gp["J"].items():
         assert "computeCoulombGradient" + cls.upper() in srcs["jg_%s.hip" % cls]
         nc, nd = [(emit.LV[x] + 1) * (emit.LV[x] + 2) // 2 for x in cls[2:]]
